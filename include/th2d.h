@@ -1,0 +1,210 @@
+/*
+ * th2d.h -- C ABI of libth2d, the MI355X (gfx950) implementation of the
+ * Thirring2D fermion hot path.
+ *
+ * Plain C types only: device pointers are `void *` / `double *`, sizes are
+ * ints, status codes are ints.  No torch, no C++ in the signatures.
+ *
+ * Device storage (all entry points below):
+ *   - lattice extents nt, nx are even; V = nt*nx, Vh = V/2;
+ *   - "eo order": site (t,x) lives at index  p*Vh + (t*nx + x)/2  with
+ *     p = (t+x)&1, i.e. all even sites in the reference's eo order
+ *     (fluctuation_determinant.c:181-192), then all odd sites;
+ *   - complex vectors are interleaved (re,im) pairs: double[2V] (FP64) or
+ *     float[2V] (FP32);
+ *   - the link angles A (hmc.c:47) are two double[V] arrays in eo order,
+ *     direction t first then x, packed as double[2V];
+ *   - "folded links" K are the per-site forward hopping factors
+ *     K_mu(x) = 1/2 eta_mu(x) s_mu(x) e^{i A_mu(x)} (s = -1 on a hop through
+ *     an antiperiodic boundary), complex, packed [dir][V] in eo order, kept
+ *     in FP64 (double[4V]) and optionally FP32 (float[4V]).
+ *
+ * Every call is asynchronous on the context's HIP stream unless it
+ * returns a host value; a non-zero return is a TH2D_E* code.
+ */
+#ifndef TH2D_H
+#define TH2D_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define TH2D_OK 0
+#define TH2D_EARG (-1)       /* bad argument (size, null, odd extent, ...)   */
+#define TH2D_EHIP (-2)       /* a HIP runtime call failed                    */
+#define TH2D_ENOCONV (-3)    /* solver hit max_iter before the tolerance     */
+#define TH2D_EBREAKDOWN (-4) /* solver breakdown: p.Ap <= 0, NaN, divergence */
+#define TH2D_ESINGULAR (-5)  /* LU met an exactly zero pivot                 */
+#define TH2D_ENOMEM (-6)     /* device allocation failed                     */
+#define TH2D_ELAUNCH (-7)    /* cooperative launch does not fit the device   */
+
+#define TH2D_PREC_F64 0
+#define TH2D_PREC_F32 1
+#define TH2D_PREC_MIXED 2    /* FP32 inner solve, FP64 reliable updates      */
+
+typedef struct th2d_ctx th2d_ctx;
+
+/* Lattice descriptor of the U(1) staggered operator (hmc.c:13-40; the
+ * reference fixes these at compile time, here they are runtime values). */
+typedef struct {
+  int nt, nx;      /* even extents                                          */
+  int anti_t;      /* 1: antiperiodic in t (hmc.c:143-158)                   */
+  int anti_x;      /* 1: antiperiodic in x (hmc.c:165-180)                   */
+  double m;        /* mass                                                  */
+  double mu;       /* chemical potential, weights e^{+-mu} on t hops         */
+} th2d_geom;
+
+/* Solver controls and diagnostics (hmc.c:341-404, vec_ops.c:261-307). */
+typedef struct {
+  double tol_rel;     /* stop at ||b - A x|| <= tol_rel ||b|| (true residual)  */
+  double tol_abs_rr;  /* or at r.r < tol_abs_rr (reference CG_ACCURACY); 0 off */
+  int max_iter;       /* cap on inner iterations (all restarts together)       */
+  int precision;      /* TH2D_PREC_F64 or TH2D_PREC_MIXED                       */
+  double delta;       /* mixed: inner FP32 solve reduces the residual by this   */
+} th2d_cg_params;
+
+typedef struct {
+  int iterations;        /* inner iterations, all restarts                     */
+  int restarts;          /* FP64 residual recomputations after the first       */
+  int status;            /* TH2D_OK / TH2D_ENOCONV / TH2D_EBREAKDOWN            */
+  double true_rel_resid; /* ||b - A x|| / ||b|| recomputed in FP64 at the end   */
+  double rr;             /* final recursive r.r (reference stopping quantity)   */
+} th2d_cg_info;
+
+/* ---- context, memory, timing ------------------------------------------ */
+int th2d_ctx_create(th2d_ctx **ctx, void *hip_stream /* NULL: default stream */);
+int th2d_ctx_destroy(th2d_ctx *ctx);
+const char *th2d_strerror(int code);
+int th2d_device_count(int *n);
+int th2d_device_props(int *num_cu, int *num_xcd, size_t *lds_per_cu, char *name, int name_len);
+
+int th2d_malloc(void **dptr, size_t bytes);
+int th2d_free(void *dptr);
+int th2d_memset(th2d_ctx *ctx, void *dptr, int value, size_t bytes);
+int th2d_h2d(th2d_ctx *ctx, void *dst, const void *src, size_t bytes);
+int th2d_d2h(th2d_ctx *ctx, void *dst, const void *src, size_t bytes);
+int th2d_d2d(th2d_ctx *ctx, void *dst, const void *src, size_t bytes);
+int th2d_sync(th2d_ctx *ctx);
+
+/* Event timing on the context stream: start, stop, elapsed ms. */
+int th2d_timer_start(th2d_ctx *ctx);
+int th2d_timer_stop(th2d_ctx *ctx, float *ms);
+
+/* Kernel-level timing: when enabled, every launch of the dominant kernel
+ * family (the CG solver kernel) is bracketed by events on the stream it is
+ * launched on and accumulated; read with th2d_kernel_time. */
+int th2d_kernel_timing(th2d_ctx *ctx, int enable);
+int th2d_kernel_time(th2d_ctx *ctx, double *total_ms, long *launches, double *total_bytes);
+
+/* ---- layout ------------------------------------------------------------ */
+/* complex vector: natural order (s = t*nx + x) <-> eo order, FP64 both. */
+int th2d_vec_to_eo(th2d_ctx *ctx, const th2d_geom *g, double *eo, const double *nat);
+int th2d_vec_from_eo(th2d_ctx *ctx, const th2d_geom *g, double *nat, const double *eo);
+/* links/momenta: natural A[V][2] <-> eo packed [dir][V]. */
+int th2d_links_to_eo(th2d_ctx *ctx, const th2d_geom *g, double *eo, const double *nat);
+int th2d_links_from_eo(th2d_ctx *ctx, const th2d_geom *g, double *nat, const double *eo);
+/* FP64 <-> FP32 copies of n complex numbers. */
+int th2d_vec_f64_to_f32(th2d_ctx *ctx, float *dst, const double *src, long n);
+int th2d_vec_f32_to_f64(th2d_ctx *ctx, double *dst, const float *src, long n);
+
+/* ---- gauge field -------------------------------------------------------- */
+/* K = 1/2 eta s e^{iA}; K32 may be NULL (hmc.c:140-180 recompute sin/cos
+ * per apply; this build folds them once per field update). */
+int th2d_fold_links(th2d_ctx *ctx, const th2d_geom *g, double *K, float *K32, const double *A);
+/* A += 2 eps mom, then refold K (hmc.c:97-101). */
+int th2d_gauge_step(th2d_ctx *ctx, const th2d_geom *g, double *A, const double *mom, double eps,
+                    double *K, float *K32);
+
+/* ---- operator ----------------------------------------------------------- */
+/* out = M in (dagger = 0, hmc.c:123-184) or out = M^dag in (dagger = 1, the
+ * true adjoint of fm_mul; the reference's fm_conjugate_mul hmc.c:188-249
+ * repeats fm_mul, defect D1).  precision TH2D_PREC_F64 (double vectors,
+ * K) or TH2D_PREC_F32 (float vectors, K32).  nrhs vectors are stored one
+ * after the other (stride 2V) and share the link reads. */
+int th2d_dslash(th2d_ctx *ctx, const th2d_geom *g, void *out, const void *in, const void *K,
+                int dagger, int precision, int nrhs);
+
+/* out = M^dag M in (FP64). */
+int th2d_mdagm(th2d_ctx *ctx, const th2d_geom *g, double *out, const double *in, const double *K,
+               double *tmp);
+
+/* ---- solver ------------------------------------------------------------- */
+/* x = (M^dag M)^{-1} b (fmdmm_invert_cg, hmc.c:341-404).  K must be folded;
+ * K32 is required for TH2D_PREC_MIXED.  x and b are FP64 eo vectors.
+ * info may be NULL. Returns info.status. */
+int th2d_cg_mdagm(th2d_ctx *ctx, const th2d_geom *g, double *x, const double *b,
+                  const double *K, const float *K32, const th2d_cg_params *p,
+                  th2d_cg_info *info);
+
+/* Tuning of the persistent solver and of the operator kernel: tile shape
+ * (tt x tx sites per workgroup tile, 0 0 = automatic), a cap on the number
+ * of co-resident workgroups (0 = as many as fit), and the Dslash variant
+ * (0 direct, 1 LDS-tiled). */
+int th2d_set_cg_tile(th2d_ctx *ctx, int tt, int tx);
+int th2d_set_cg_max_blocks(th2d_ctx *ctx, int n);
+int th2d_set_dslash_variant(int variant);
+
+/* ---- force -------------------------------------------------------------- */
+/* mom -= eps * dS/dA for S = Nf/g sum(1-cos A) + psi^dag (M^dag M)^{-1} psi
+ *                             + Re st^dag M^dag st
+ * given chi = (M^dag M)^{-1} psi (momentum_step, hmc.c:504-665).  st may
+ * be NULL (its force vanishes at mu = 0).  If force_out != NULL it receives
+ * dS/dA in eo packed [dir][V] order and mom is left unchanged. */
+int th2d_force(th2d_ctx *ctx, const th2d_geom *g, double *mom, const double *A, const double *K,
+               const double *chi, const double *st, double eps, double gcoup, int nf,
+               double *force_out);
+
+/* ---- reductions (results on the host; these synchronise) --------------- */
+int th2d_dot(th2d_ctx *ctx, long n_complex, const double *a, const double *b, double *re,
+             double *im);
+int th2d_norm2(th2d_ctx *ctx, long n_real, const double *a, double *out);
+/* Nf/g sum(1 - cos A) (calc_gauge_action, hmc.c:73-79). */
+int th2d_gauge_action(th2d_ctx *ctx, const th2d_geom *g, const double *A, double gcoup, int nf,
+                      double *out);
+
+/* ---- HMC ---------------------------------------------------------------- */
+/* Parameters of one trajectory (update_gauge, hmc.c:671-746). */
+typedef struct {
+  double g;          /* coupling                                        */
+  int nf;            /* Nf of the gauge action (hmc.c:28)                */
+  int nsteps;        /* leapfrog steps (hmc.c:708)                       */
+  double traj_len;   /* trajectory length (hmc.c:709)                     */
+  int phys_norm;     /* 0: reference noise normalisation (D4/D5); 1: fixed */
+  th2d_cg_params cg; /* solver used for every inversion                 */
+} th2d_hmc_params;
+
+typedef struct {
+  double dS;          /* H_end - H_start                              */
+  double acts[8];     /* Sg0, Spf0, Smd0, Smom0, Sg1, Spf1, Smd1, Smom1  */
+  int accepted;
+  int cg_iterations;  /* inner iterations summed over the trajectory   */
+  int cg_solves;
+  double max_true_rel_resid;
+  int status;
+} th2d_hmc_result;
+
+/* One trajectory on device-resident fields.  A (eo, FP64) is the current
+ * gauge field and is replaced on accept.  The noise comes either from the
+ * host (xi, mom, st: natural-order host arrays drawn from the reference's
+ * MT stream; u = the accept draw) or, when xi == NULL, from the device
+ * counter-based generator keyed by (seed, counter). */
+int th2d_hmc_trajectory(th2d_ctx *ctx, const th2d_geom *geo, const th2d_hmc_params *hp,
+                        double *A, const double *xi_host, const double *mom_host,
+                        const double *st_host, double u_accept, unsigned long long seed,
+                        unsigned long long counter, th2d_hmc_result *res);
+
+/* Gaussian N(0,1) link angles from the device generator (configs: A~N(0,1)). */
+int th2d_random_links(th2d_ctx *ctx, const th2d_geom *g, double *A, unsigned long long seed);
+
+/* One pure-gauge Metropolis sweep with host-supplied uniform draws, two
+ * per link in natural link order (update_puregauge_hb, hmc.c:82-93). */
+int th2d_puregauge_sweep(th2d_ctx *ctx, const th2d_geom *g, double *A, const double *draws_dev,
+                         double gcoup, int nf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,234 @@
+"""GPU parity of the HMC hot path (hmc.c) against the CPU oracle.
+
+Tolerances: FP64 operator and force <= 1e-12 relative (max |diff| / max
+|ref|); FP32 operator <= 2e-6 relative; solver results are compared through
+the true residual (<= the requested tolerance) and against the oracle's
+solution at the solver tolerance.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(4, 4), (8, 8), (16, 16), (8, 32), (32, 8), (48, 48), (64, 32)]
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def hmc():
+    from thirring2d_amd import hmc as H
+    return H
+
+
+def _setup(H, O, nt, nx, seed, m=0.1, g=0.5, mu=0.0, anti_t=1, anti_x=1):
+    rng = np.random.default_rng(seed)
+    V = nt * nx
+    A = rng.normal(size=(V, 2))
+    lat = H.StaggeredLattice(nt, nx, m=m, g=g, mu=mu, anti_t=anti_t, anti_x=anti_x)
+    lat.set_links(A)
+    L = O.lattice(nt, nx, m=m, g=g, mu=mu, anti_t=anti_t, anti_x=anti_x)
+    v = rng.normal(size=V) + 1j * rng.normal(size=V)
+    return lat, L, A, v, rng
+
+
+@pytest.mark.parametrize("nt,nx", SIZES)
+@pytest.mark.parametrize("mu", [0.0, 0.3])
+def test_fm_mul_and_adjoint(hmc, oracle, nt, nx, mu):
+    lat, L, A, v, _ = _setup(hmc, oracle, nt, nx, 11 + nt + nx, mu=mu)
+    assert _rel(lat.fm_mul(v), oracle.fm_mul(L, A, v)) <= 1e-12
+    assert _rel(lat.fm_conjugate_mul(v), oracle.fm_dag_mul(L, A, v)) <= 1e-12
+
+
+@pytest.mark.parametrize("anti_t,anti_x", [(1, 1), (1, 0), (0, 0)])
+def test_boundary_conditions(hmc, oracle, anti_t, anti_x):
+    lat, L, A, v, _ = _setup(hmc, oracle, 8, 12, 5, anti_t=anti_t, anti_x=anti_x)
+    assert _rel(lat.fm_mul(v), oracle.fm_mul(L, A, v)) <= 1e-12
+
+
+def test_links_round_trip(hmc, oracle):
+    lat, L, A, v, _ = _setup(hmc, oracle, 16, 8, 3)
+    np.testing.assert_array_equal(lat.get_links(), A)
+
+
+@pytest.mark.parametrize("nt,nx", [(8, 8), (32, 32), (64, 64)])
+def test_dslash_fp32(hmc, oracle, nt, nx):
+    lat, L, A, v, _ = _setup(hmc, oracle, nt, nx, 7)
+    for dag, ref in ((0, oracle.fm_mul(L, A, v)), (1, oracle.fm_dag_mul(L, A, v))):
+        assert _rel(lat.apply_f32(v, dag), ref) <= 2e-6
+
+
+def test_dslash_lds_variant_matches_direct(hmc, oracle):
+    from thirring2d_amd import _native as N
+    lat, L, A, v, _ = _setup(hmc, oracle, 36, 70, 9)
+    ref = oracle.fm_mul(L, A, v)
+    try:
+        N.lib().th2d_set_dslash_variant(1)
+        out_lds = lat.fm_mul(v)
+        outd_lds = lat.fm_conjugate_mul(v)
+    finally:
+        N.lib().th2d_set_dslash_variant(0)
+    assert _rel(out_lds, ref) <= 1e-12
+    assert _rel(outd_lds, oracle.fm_dag_mul(L, A, v)) <= 1e-12
+
+
+@pytest.mark.parametrize("nt,nx", [(4, 4), (8, 8), (16, 16), (24, 40), (64, 64)])
+def test_cg_fp64(hmc, oracle, nt, nx):
+    lat, L, A, b, _ = _setup(hmc, oracle, nt, nx, 21, m=0.1)
+    x, info = lat.fmdm_invert_cg(b, hmc.cg_params(tol_rel=1e-12))
+    assert info.status == 0
+    r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-12
+    assert info.true_rel_resid <= 1e-12
+    xo, it, st = oracle.cg_mdm(L, A, b, tol_rel=1e-13)
+    assert st == 0
+    assert _rel(x, xo) <= 1e-9
+    assert abs(info.iterations - it) <= max(3, it // 10)
+
+
+@pytest.mark.parametrize("nt,nx", [(16, 16), (64, 64), (128, 128)])
+def test_cg_mixed(hmc, oracle, nt, nx):
+    from thirring2d_amd import _native as N
+    lat, L, A, b, _ = _setup(hmc, oracle, nt, nx, 31, m=0.05)
+    x, info = lat.fmdm_invert_cg(b, hmc.cg_params(tol_rel=1e-10, precision=N.PREC_MIXED))
+    assert info.status == 0
+    r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
+    rel = np.linalg.norm(r) / np.linalg.norm(b)
+    assert rel <= 1e-10
+    assert abs(rel - info.true_rel_resid) <= 1e-3 * rel + 1e-15
+
+
+def test_cg_reference_criterion(hmc, oracle):
+    """Absolute stop r.r < 1e-30 on the recursive residual (hmc.c:34,381)."""
+    lat, L, A, b, _ = _setup(hmc, oracle, 16, 16, 41, m=0.1)
+    x, info = lat.fmdm_invert_cg(b, hmc.cg_params(tol_rel=0.0, tol_abs_rr=1e-30))
+    xo, it, st = oracle.cg_mdm(L, A, b, tol_abs_rr=1e-30, tol_rel=0.0)
+    assert info.status == 0 and st == 0
+    assert info.rr < 1e-30
+    assert _rel(x, xo) <= 1e-12
+    assert abs(info.iterations - it) <= 3
+
+
+@pytest.mark.parametrize("tile", [(4, 8), (8, 8), (8, 16), (16, 16), (16, 32)])
+def test_cg_tiles(hmc, oracle, tile):
+    """Every tile shape (several tiles per block included) solves the same system."""
+    from thirring2d_amd import _native as N
+    lat, L, A, b, _ = _setup(hmc, oracle, 40, 48, 51, m=0.2)
+    N.lib().th2d_set_cg_tile(lat.ctx.ptr, *tile)
+    try:
+        x, info = lat.fmdm_invert_cg(b, hmc.cg_params(tol_rel=1e-12))
+    finally:
+        N.lib().th2d_set_cg_tile(lat.ctx.ptr, 0, 0)
+    r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-12
+
+
+def test_cg_several_tiles_per_block(hmc, oracle):
+    """Fewer workgroups than tiles: links are restaged per tile and step."""
+    from thirring2d_amd import _native as N
+    lat, L, A, b, _ = _setup(hmc, oracle, 64, 64, 53, m=0.2)
+    N.lib().th2d_set_cg_tile(lat.ctx.ptr, 8, 8)
+    N.lib().th2d_set_cg_max_blocks(lat.ctx.ptr, 7)
+    try:
+        x, info = lat.fmdm_invert_cg(b, hmc.cg_params(tol_rel=1e-12))
+    finally:
+        N.lib().th2d_set_cg_tile(lat.ctx.ptr, 0, 0)
+        N.lib().th2d_set_cg_max_blocks(lat.ctx.ptr, 0)
+    r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-12
+
+
+@pytest.mark.parametrize("mu", [0.0, 0.25])
+def test_force_parity(hmc, oracle, mu):
+    lat, L, A, v, rng = _setup(hmc, oracle, 16, 16, 61, m=0.1, g=0.8, mu=mu)
+    psi = oracle.fm_dag_mul(L, A, v)
+    st = rng.normal(size=L.nt * L.nx) + 1j * rng.normal(size=L.nt * L.nx)
+    _, F_ref, _ = oracle.momentum_step(L, np.zeros((256, 2)), A, psi, st, 0.1, tol_rel=1e-14)
+    chi, info = lat.fmdm_invert_cg(psi, hmc.cg_params(tol_rel=1e-14))
+    F = lat.force(chi, st)
+    assert _rel(F, F_ref) <= 1e-10
+
+
+def test_force_finite_differences(hmc, oracle):
+    """Central differences h = 1e-6 of S_g + S_pf on links incl. wrap-around."""
+    lat, L, A, v, rng = _setup(hmc, oracle, 12, 12, 71, m=0.1, g=0.8)
+    psi = oracle.fm_dag_mul(L, A, v)
+    chi, _ = lat.fmdm_invert_cg(psi, hmc.cg_params(tol_rel=1e-14))
+    F = lat.force(chi)
+    h = 1e-6
+    links = [(0, 0), (11, 1), (143, 0), (132, 0), (143, 1), (70, 1), (5, 0), (99, 1)]
+    for s, d in links:
+        Sp = []
+        for sg in (1, -1):
+            Ap = A.copy()
+            Ap[s, d] += sg * h
+            Sp.append(oracle.pseudofermion_action(L, Ap, psi, tol_rel=1e-14)[0]
+                      + oracle.gauge_action(L, Ap))
+        fd = (Sp[0] - Sp[1]) / (2 * h)
+        assert abs(F[s, d] - fd) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_gauge_step_and_action(hmc, oracle):
+    lat, L, A, v, rng = _setup(hmc, oracle, 16, 8, 81)
+    mom = rng.normal(size=(128, 2))
+    lat.gauge_step(mom, 0.05)
+    np.testing.assert_allclose(lat.get_links(), oracle.gauge_step(L, A, mom, 0.05), rtol=0,
+                               atol=1e-15)
+    assert abs(lat.calc_gauge_action() - oracle.gauge_action(L, lat.get_links())) <= 1e-11
+
+
+@pytest.mark.parametrize("nsteps", [10, 40])
+def test_trajectory_parity(hmc, oracle, nsteps):
+    """Same MT noise, same field: dH, the actions and the accept decision of
+    three consecutive trajectories agree with the oracle (16x16, m=0.1, g=0.5)."""
+    nt = nx = 16
+    V = nt * nx
+    L = oracle.lattice(nt, nx, m=0.1, g=0.5)
+    A0 = np.random.default_rng(1001).normal(size=(V, 2))
+    lat = hmc.StaggeredLattice(nt, nx, m=0.1, g=0.5)
+    lat.set_links(A0)
+    r_or = oracle.Rng(2001, warmup=543210)
+    r_gpu = oracle.Rng(2001, warmup=543210)
+    A = A0.copy()
+    for _ in range(3):
+        # noise for the device in the reference draw order
+        xi = np.array([complex(*_pair(r_gpu)) for _ in range(V)])
+        mom = np.array([_pair(r_gpu) for _ in range(V)])
+        st = np.array([complex(*_pair(r_gpu)) for _ in range(V)])
+        u = r_gpu.next()
+        res = lat.update_gauge(nsteps=nsteps, noise=(xi, mom, st), u_accept=u,
+                               params=hmc.cg_params(tol_rel=1e-12))
+        acc, dS, acts, its, A = oracle.update_gauge(L, r_or, A, nsteps=nsteps, tol_rel=1e-12)
+        assert res.status == 0
+        # nsteps = 10 (dt = 0.1) is beyond the leapfrog's stability limit at
+        # m = 0.1 (dH ~ 1e5): round-off differences grow along the
+        # trajectory, so the bound is looser there than in the stable case.
+        tol = 1e-6 if nsteps == 10 else 1e-9
+        assert abs(res.dS - dS) <= tol * max(1.0, abs(dS))
+        np.testing.assert_allclose(np.array(res.acts), acts, rtol=tol, atol=tol)
+        assert res.accepted == acc
+        np.testing.assert_allclose(lat.get_links(), A, rtol=0, atol=1e-9)
+
+
+def _pair(r):
+    x1, x2 = r.next(), r.next()
+    rad = np.sqrt(-2 * np.log(x1))
+    return rad * np.cos(2 * np.pi * x2), rad * np.sin(2 * np.pi * x2)
+
+
+def test_trajectory_mixed_vs_fp64(hmc):
+    """Mixed-precision trajectory: dH within 1e-6 of the FP64 one on the same
+    field and device noise (residual 1e-10)."""
+    from thirring2d_amd import _native as N
+    out = []
+    for prec in (N.PREC_F64, N.PREC_MIXED):
+        lat = hmc.StaggeredLattice(64, 64, m=0.1, g=0.5)
+        lat.random_links(1002)
+        res = lat.update_gauge(nsteps=10, params=hmc.cg_params(tol_rel=1e-10, precision=prec),
+                               seed=77, counter=0)
+        assert res.status == 0
+        assert res.max_true_rel_resid <= 1e-10
+        out.append(res.dS)
+    assert abs(out[0] - out[1]) <= 1e-6 * max(1.0, abs(out[0]))

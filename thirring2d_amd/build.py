@@ -1,0 +1,60 @@
+"""Build libth2d.so in-tree for gfx950 with hipcc (no JIT, no cache dir).
+
+    python -m thirring2d_amd.build
+"""
+from __future__ import annotations
+
+import glob
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libth2d.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# Packed FP32 instructions are disabled for every HIP source.
+HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-function"]
+
+
+def sources():
+    return sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+
+
+def c_sources():
+    return sorted(glob.glob(os.path.join(CSRC, "*.c")))
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every HIP/C source and link libth2d.so; returns its path."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src in c_sources():
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        cmd = ["gcc", "-O2", "-std=c99", "-fPIC", "-Wall", "-Wextra", "-c", src, "-o", obj]
+        _run(cmd, verbose)
+        objs.append(obj)
+    tmp = OUT + ".tmp"
+    cmd = [HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + sources() + objs
+    _run(cmd, verbose)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in p.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if p.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
+    if verbose and out.strip():
+        print(out)
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))
