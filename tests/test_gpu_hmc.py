@@ -125,6 +125,19 @@ def test_cg_tiles(hmc, oracle, tile):
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-12
 
 
+@pytest.mark.parametrize("n,prec", [(256, 0), (256, 2), (512, 2)])
+def test_cg_large_lattice(hmc, oracle, n, prec):
+    """Regression: the solver's scratch block holds eight vectors; with seven
+    the partial sums and the stop flag overran the allocation by one vector,
+    which only left the allocator's padding at large lattices."""
+    from thirring2d_amd import _native as N
+    lat, L, A, b, _ = _setup(hmc, oracle, n, n, 91, m=0.2)
+    x, info = lat.fmdm_invert_cg(b, hmc.cg_params(tol_rel=1e-10, precision=prec))
+    assert info.status == 0
+    r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
+
+
 def test_cg_several_tiles_per_block(hmc, oracle):
     """Fewer workgroups than tiles: links are restaged per tile and step."""
     from thirring2d_amd import _native as N
@@ -219,16 +232,18 @@ def _pair(r):
 
 
 def test_trajectory_mixed_vs_fp64(hmc):
-    """Mixed-precision trajectory: dH within 1e-6 of the FP64 one on the same
-    field and device noise (residual 1e-10)."""
+    """Mixed-precision trajectory against the FP64 one on the same field and
+    device noise, in the stable regime (64x64, m = 0.1, 40 x dt = 0.025,
+    residual 1e-10): |dH_mixed - dH_fp64| <= 1e-5 + 1e-6 |dH|."""
     from thirring2d_amd import _native as N
     out = []
     for prec in (N.PREC_F64, N.PREC_MIXED):
         lat = hmc.StaggeredLattice(64, 64, m=0.1, g=0.5)
         lat.random_links(1002)
-        res = lat.update_gauge(nsteps=10, params=hmc.cg_params(tol_rel=1e-10, precision=prec),
+        res = lat.update_gauge(nsteps=40, params=hmc.cg_params(tol_rel=1e-10, precision=prec),
                                seed=77, counter=0)
         assert res.status == 0
         assert res.max_true_rel_resid <= 1e-10
         out.append(res.dS)
-    assert abs(out[0] - out[1]) <= 1e-6 * max(1.0, abs(out[0]))
+    assert abs(out[0]) < 10.0
+    assert abs(out[0] - out[1]) <= 1e-5 + 1e-6 * abs(out[0])

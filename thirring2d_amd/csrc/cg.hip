@@ -1,28 +1,42 @@
-// cg.hip -- persistent conjugate-gradient solver for M^dag M on gfx950
+// cg.hip -- conjugate-gradient solver for M^dag M on gfx950
 // (fmdm_invert_cg, hmc.c:341-404; same Krylov iterates, different
 // schedule).
 //
 // Schedule: Chronopoulos-Gear CG, whose two inner products (r.r, r.Ar) are
-// taken together, so one grid-wide barrier per iteration suffices.  Each
-// workgroup owns TT x TX tiles of the lattice.  Per iteration a tile
+// taken together, so one grid-wide synchronisation per iteration
+// suffices; it is the boundary between two launches of one fused step
+// kernel.  Each workgroup owns TT x TX tiles of the lattice.  Iteration j
+// (one launch) does, per tile:
 //   1. reads r_j, w_j = A r_j and s_{j-1} on the tile plus a 2-site halo,
 //      forms s_j = w_j + beta s_{j-1} and r_{j+1} = r_j - alpha s_j there
 //      (the halo values are recomputed, not exchanged);
 //   2. applies M to r_{j+1} on the tile plus a 1-site halo (LDS);
 //   3. applies M^dag to that on the tile: w_{j+1} = A r_{j+1};
 //   4. updates p_j = r_j + beta p_{j-1}, x_{j+1} = x_j + alpha p_j in place;
-//   5. writes its partial (r.r, r.w) and joins the barrier.
-// r, s, w are double-buffered by iteration parity; p, x are only touched by
-// their owner.  Every workgroup adds the partials in the same order, so all
-// take bitwise the same alpha/beta and stop decision.  The folded links of
-// a tile stay in LDS for the whole solve when each workgroup owns one tile.
+//   5. writes its partial (r.r, r.w).
+// The next launch starts by adding all partials in a fixed order (every
+// block the same way, so all take bitwise the same alpha/beta and stop
+// decision).  r, s, w, the partials and the scalar state are
+// double-buffered by iteration parity; p, x are only touched by their
+// owner.  The host enqueues launches in chunks and reads the stop flag once
+// per chunk; launches after the stop return at once.
 #include "th2d_internal.h"
 
 #include <algorithm>
 #include <cmath>
-#include <vector>
 
 namespace {
+
+struct CgState {
+  double gam_old, alpha_old;
+  int j, pad;
+};
+
+struct CgResult {
+  int iters, status;
+  double rr;
+  int done, pad;
+};
 
 template <typename T> struct CgArgs {
   Geo G;
@@ -32,33 +46,16 @@ template <typename T> struct CgArgs {
   c2<T> *r[2], *s[2], *w[2];
   const c2<T> *K;  // [2][V]
   double *part;    // [2][gridDim][2]
-  unsigned *bar;   // zeroed before launch
+  CgState *state;  // [2]
+  CgResult *res;   // result block and sticky stop flag
   double stop_rr;
   int max_iter;
-  int xcd_map;
-  int *out_iters;
-  double *out_rr;
-  int *out_status;
 };
 
 __device__ __forceinline__ int wrapc(int v, int n)
 {
   v %= n;
   return v < 0 ? v + n : v;
-}
-
-__device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned &target)
-{
-  target += gridDim.x;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    atomicAdd(bar, 1u);
-    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
-      __builtin_amdgcn_s_sleep(1);
-    __threadfence();
-  }
-  __syncthreads();
 }
 
 template <typename T> struct TileCtx {
@@ -128,23 +125,10 @@ __device__ void apply_and_reduce(const CgArgs<T> &a, const TileCtx<T> &c, int t0
 }
 
 template <typename T>
-__device__ __forceinline__ int tile_of_block(const CgArgs<T> &a, int k)
-{
-  // k-th tile of this block.  With one tile per block and a grid that is a
-  // multiple of 8, blocks of one XCD (round-robin dispatch: XCD = block % 8)
-  // take a contiguous band of tiles so that most halo reads stay in that
-  // XCD's L2.
-  int nb = gridDim.x, b = blockIdx.x;
-  if (a.xcd_map && a.ntiles == nb && (nb & 7) == 0) return (b & 7) * (nb >> 3) + (b >> 3);
-  return b + k * nb;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_cg(CgArgs<T> a)
+__device__ __forceinline__ TileCtx<T> tile_ctx(const CgArgs<T> &a)
 {
   using V2 = c2<T>;
   extern __shared__ double lds_raw[];
-  const Geo &G = a.G;
   TileCtx<T> c;
   c.W2 = a.TX + 4;
   c.W1 = a.TX + 2;
@@ -155,16 +139,31 @@ __global__ void __launch_bounds__(256) k_cg(CgArgs<T> a)
   c.LKt = c.Y1 + c.E1;
   c.LKx = c.LKt + c.E2;
   c.red = (double *)(c.LKx + c.E2);
+  return c;
+}
 
-  const int nb = gridDim.x;
-  const bool resident = a.ntiles <= nb;
-  unsigned target = 0;
+template <typename T>
+__device__ __forceinline__ void write_partials(const CgArgs<T> &a, const TileCtx<T> &c, int slot,
+                                               double gacc, double dacc)
+{
+  double gs = block_sum(gacc, c.red);
+  __syncthreads();
+  double ds = block_sum(dacc, c.red + 16);
+  if (threadIdx.x == 0) {
+    a.part[(slot * gridDim.x + blockIdx.x) * 2 + 0] = gs;
+    a.part[(slot * gridDim.x + blockIdx.x) * 2 + 1] = ds;
+  }
+}
 
-  // ---- init: r_0 = b, w_0 = A b, s_{-1} = p_{-1} = x_0 = 0 ----
+// r_0 = b, w_0 = A b, s_{-1} = p_{-1} = x_0 = 0; partials of slot 0.
+template <typename T>
+__global__ void __launch_bounds__(256) k_cg_init(CgArgs<T> a)
+{
+  using V2 = c2<T>;
+  const Geo &G = a.G;
+  TileCtx<T> c = tile_ctx(a);
   double gacc = 0.0, dacc = 0.0;
-  for (int k = 0;; k++) {
-    int tau = tile_of_block(a, k);
-    if (tau >= a.ntiles || (resident && k > 0)) break;
+  for (int tau = blockIdx.x; tau < a.ntiles; tau += gridDim.x) {
     int t0 = (tau / a.tiles_x) * a.TT, x0 = (tau % a.tiles_x) * a.TX;
     __syncthreads();
     load_links(a, c, t0, x0);
@@ -186,97 +185,97 @@ __global__ void __launch_bounds__(256) k_cg(CgArgs<T> a)
     __syncthreads();
     apply_and_reduce(a, c, t0, x0, a.w[0], gacc, dacc);
   }
-  {
-    double gs = block_sum(gacc, c.red);
-    __syncthreads();
-    double ds = block_sum(dacc, c.red + 16);
-    if (threadIdx.x == 0) {
-      a.part[(0 * nb + blockIdx.x) * 2 + 0] = gs;
-      a.part[(0 * nb + blockIdx.x) * 2 + 1] = ds;
-    }
+  write_partials(a, c, 0, gacc, dacc);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.state[0].gam_old = 1.0;
+    a.state[0].alpha_old = 1.0;
+    a.state[0].j = 0;
+    a.res->done = 0;
+    a.res->iters = 0;
+    a.res->status = TH2D_OK;
+    a.res->rr = 0.0;
   }
-  grid_barrier(a.bar, target);
+}
 
-  // every block sums the partials in the same order
-  auto reduce_slot = [&](int slot, double &g, double &d) {
-    double pg = 0.0, pd = 0.0;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
-      pg += a.part[(slot * nb + k) * 2 + 0];
-      pd += a.part[(slot * nb + k) * 2 + 1];
+// Iteration j = state[slot].j; reads the slot's buffers, writes slot^1.
+template <typename T>
+__global__ void __launch_bounds__(256) k_cg_step(CgArgs<T> a, int slot)
+{
+  using V2 = c2<T>;
+  const Geo &G = a.G;
+  if (a.res->done) return;
+  TileCtx<T> c = tile_ctx(a);
+  const int nb = gridDim.x, nxt = slot ^ 1;
+  // every block adds the partials of the previous launch in one order
+  double pg = 0.0, pd = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+    pg += a.part[(slot * nb + k) * 2 + 0];
+    pd += a.part[(slot * nb + k) * 2 + 1];
+  }
+  double gs = block_sum(pg, c.red);
+  __syncthreads();
+  double ds = block_sum(pd, c.red + 16);
+  if (threadIdx.x == 0) {
+    c.red[32] = gs;
+    c.red[33] = ds;
+  }
+  __syncthreads();
+  const double gam = c.red[32], del = c.red[33];
+  const CgState st = a.state[slot];
+  const int j = st.j;
+  bool stop = true;
+  int status = TH2D_OK;
+  double beta = 0.0, alpha = 0.0;
+  if (gam < a.stop_rr) status = TH2D_OK;
+  else if (j >= a.max_iter) status = TH2D_ENOCONV;
+  else if (!(del > 0.0) || !isfinite(gam)) status = TH2D_EBREAKDOWN;
+  else {
+    beta = j == 0 ? 0.0 : gam / st.gam_old;
+    alpha = j == 0 ? gam / del : gam / (del - beta * gam / st.alpha_old);
+    if (!(alpha > 0.0) || !isfinite(alpha)) status = TH2D_EBREAKDOWN;
+    else stop = false;
+  }
+  if (stop) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.res->iters = j;
+      a.res->status = status;
+      a.res->rr = gam;
+      a.res->done = 1;
     }
-    double gs = block_sum(pg, c.red);
-    __syncthreads();
-    double ds = block_sum(pd, c.red + 16);
-    if (threadIdx.x == 0) {
-      c.red[32] = gs;
-      c.red[33] = ds;
-    }
-    __syncthreads();
-    g = c.red[32];
-    d = c.red[33];
-    __syncthreads();
-  };
-
-  double gam, del;
-  reduce_slot(0, gam, del);
-  double gam_old = 1.0, alpha_old = 1.0;
-  int status = TH2D_OK, j = 0;
-  for (;; j++) {
-    if (gam < a.stop_rr) break;
-    if (j >= a.max_iter) { status = TH2D_ENOCONV; break; }
-    if (!(del > 0.0) || !isfinite(gam)) { status = TH2D_EBREAKDOWN; break; }
-    double beta = j == 0 ? 0.0 : gam / gam_old;
-    double alpha = j == 0 ? gam / del : gam / (del - beta * gam / alpha_old);
-    if (!(alpha > 0.0) || !isfinite(alpha)) { status = TH2D_EBREAKDOWN; break; }
-    const int cur = j & 1, nxt = cur ^ 1;
-    const T ta = (T)alpha, tb = (T)beta;
-    gacc = 0.0;
-    dacc = 0.0;
-    for (int k = 0;; k++) {
-      int tau = tile_of_block(a, k);
-      if (tau >= a.ntiles || (resident && k > 0)) break;
-      int t0 = (tau / a.tiles_x) * a.TT, x0 = (tau % a.tiles_x) * a.TX;
-      __syncthreads();
-      if (!resident) load_links(a, c, t0, x0);
-      for (int e = threadIdx.x; e < c.E2; e += blockDim.x) {
-        int et = e / c.W2, ex = e - et * c.W2;
-        int tg = t0 - 2 + et, xg = x0 - 2 + ex;
-        int i = eo_index(G, wrapc(tg, G.nt), wrapc(xg, G.nx));
-        V2 rv = a.r[cur][i], wv = a.w[cur][i], sv = a.s[cur][i];
-        V2 sn = cadd(wv, cscale(sv, tb));
-        V2 rn = csub(rv, cscale(sn, ta));
-        c.R2[e] = rn;
-        bool own = et >= 2 && et < a.TT + 2 && ex >= 2 && ex < a.TX + 2 && tg < G.nt && xg < G.nx;
-        if (own) {
-          V2 pn = cadd(rv, cscale(a.p[i], tb));
-          a.p[i] = pn;
-          a.x[i] = cadd(a.x[i], cscale(pn, ta));
-          a.s[nxt][i] = sn;
-          a.r[nxt][i] = rn;
-        }
-      }
-      __syncthreads();
-      apply_and_reduce(a, c, t0, x0, a.w[nxt], gacc, dacc);
-    }
-    {
-      double gs = block_sum(gacc, c.red);
-      __syncthreads();
-      double ds = block_sum(dacc, c.red + 16);
-      if (threadIdx.x == 0) {
-        a.part[(nxt * nb + blockIdx.x) * 2 + 0] = gs;
-        a.part[(nxt * nb + blockIdx.x) * 2 + 1] = ds;
-      }
-    }
-    grid_barrier(a.bar, target);
-    gam_old = gam;
-    alpha_old = alpha;
-    reduce_slot(nxt, gam, del);
+    return;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *a.out_iters = j;
-    *a.out_rr = gam;
-    *a.out_status = status;
+    a.state[nxt].gam_old = gam;
+    a.state[nxt].alpha_old = alpha;
+    a.state[nxt].j = j + 1;
   }
+  const T ta = (T)alpha, tb = (T)beta;
+  double gacc = 0.0, dacc = 0.0;
+  for (int tau = blockIdx.x; tau < a.ntiles; tau += gridDim.x) {
+    int t0 = (tau / a.tiles_x) * a.TT, x0 = (tau % a.tiles_x) * a.TX;
+    __syncthreads();
+    load_links(a, c, t0, x0);
+    for (int e = threadIdx.x; e < c.E2; e += blockDim.x) {
+      int et = e / c.W2, ex = e - et * c.W2;
+      int tg = t0 - 2 + et, xg = x0 - 2 + ex;
+      int i = eo_index(G, wrapc(tg, G.nt), wrapc(xg, G.nx));
+      V2 rv = a.r[slot][i], wv = a.w[slot][i], sv = a.s[slot][i];
+      V2 sn = cadd(wv, cscale(sv, tb));
+      V2 rn = csub(rv, cscale(sn, ta));
+      c.R2[e] = rn;
+      bool own = et >= 2 && et < a.TT + 2 && ex >= 2 && ex < a.TX + 2 && tg < G.nt && xg < G.nx;
+      if (own) {
+        V2 pn = cadd(rv, cscale(a.p[i], tb));
+        a.p[i] = pn;
+        a.x[i] = cadd(a.x[i], cscale(pn, ta));
+        a.s[nxt][i] = sn;
+        a.r[nxt][i] = rn;
+      }
+    }
+    __syncthreads();
+    apply_and_reduce(a, c, t0, x0, a.w[nxt], gacc, dacc);
+  }
+  write_partials(a, c, nxt, gacc, dacc);
 }
 
 struct TileChoice {
@@ -315,24 +314,26 @@ template <typename T> size_t lds_bytes(int tt, int tx)
   return sizeof(c2<T>) * (3 * e2 + e1) + sizeof(double) * 48;
 }
 
-// Scratch layout of one solve of precision T: 7 vectors + partials +
-// barrier + outputs.
+// Scratch layout of one solve of precision T: 7 vectors, partials,
+// scalar state and the result block.
 template <typename T> struct SolveBufs {
   c2<T> *x, *p, *r[2], *s[2], *w[2];
   double *part;
-  unsigned *bar;
-  int *iters;
-  double *rr;
-  int *status;
+  CgState *state;
+  CgResult *res;
 };
 
 template <typename T>
-int solve_bufs(th2d_ctx *ctx, th2d_buf &buf, long V, int max_blocks, SolveBufs<T> &o)
+int solve_bufs(th2d_ctx *ctx, th2d_buf &buf, long V, int nblocks, SolveBufs<T> &o)
 {
+  (void)ctx;
   size_t vec = sizeof(c2<T>) * (size_t)V;
   vec = (vec + 255) & ~(size_t)255;
-  size_t part = sizeof(double) * 4 * (size_t)max_blocks;
-  size_t bytes = 7 * vec + part + 256;
+  size_t part = sizeof(double) * 4 * (size_t)nblocks;
+  part = (part + 255) & ~(size_t)255;
+  // x, p, r[2], s[2], w[2]: eight vectors, then partials, state, result
+  const size_t nvec = 8;
+  size_t bytes = nvec * vec + part + 256;
   TH2D_TRY(th2d_buf_reserve(buf, bytes));
   char *p = (char *)buf.ptr;
   o.x = (c2<T> *)p;
@@ -349,10 +350,9 @@ int solve_bufs(th2d_ctx *ctx, th2d_buf &buf, long V, int max_blocks, SolveBufs<T
   }
   o.part = (double *)p;
   p += part;
-  o.bar = (unsigned *)p;          // 16 bytes, cleared before each launch
-  o.iters = (int *)(p + 16);      // result block: iters, rr, status
-  o.rr = (double *)(p + 24);
-  o.status = (int *)(p + 32);
+  o.state = (CgState *)p;          // 2 x 24 bytes
+  o.res = (CgResult *)(p + 64);    // 24 bytes
+  if ((size_t)((char *)(o.res + 1) - (char *)buf.ptr) > bytes) return TH2D_EARG;
   return TH2D_OK;
 }
 
@@ -362,8 +362,8 @@ struct InnerResult {
   int status;
 };
 
-// One persistent solve A y = rhs from y = 0 with recursive stop rr < stop.
-// rhs and y are precision-T eo vectors; y may alias the internal x.
+// One solve A y = rhs from y = 0 with recursive stop rr < stop; y is the
+// internal x of the scratch block, returned in *xout.
 template <typename T>
 int inner_solve(th2d_ctx *ctx, const Geo &G, th2d_buf &buf, const c2<T> *rhs, const c2<T> *K,
                 double stop_rr, int max_iter, c2<T> **xout, InnerResult *res)
@@ -373,15 +373,10 @@ int inner_solve(th2d_ctx *ctx, const Geo &G, th2d_buf &buf, const c2<T> *rhs, co
   int tiles_x = (G.nx + tc.tx - 1) / tc.tx;
   int ntiles = ((G.nt + tc.tt - 1) / tc.tt) * tiles_x;
   size_t lds = lds_bytes<T>(tc.tt, tc.tx);
-  int per_cu = 0;
-  TH2D_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_cg<T>, 256,
-                                                         lds));
-  if (per_cu < 1) return TH2D_ELAUNCH;
-  int max_blocks = per_cu * ctx->num_cu;
-  if (ctx->cg_max_blocks > 0) max_blocks = std::min(max_blocks, ctx->cg_max_blocks);
-  int nblocks = std::min(ntiles, max_blocks);
+  // one tile per block up to a cap of blocks; more tiles loop in a block
+  int nblocks = std::min(ntiles, ctx->cg_max_blocks > 0 ? ctx->cg_max_blocks : 4 * ctx->num_cu);
   SolveBufs<T> sb;
-  TH2D_TRY(solve_bufs<T>(ctx, buf, V, std::max(nblocks, 1), sb));
+  TH2D_TRY(solve_bufs<T>(ctx, buf, V, nblocks, sb));
   CgArgs<T> a;
   a.G = G;
   a.TT = tc.tt;
@@ -398,44 +393,53 @@ int inner_solve(th2d_ctx *ctx, const Geo &G, th2d_buf &buf, const c2<T> *rhs, co
   }
   a.K = K;
   a.part = sb.part;
-  a.bar = sb.bar;
+  a.state = sb.state;
+  a.res = sb.res;
   a.stop_rr = stop_rr;
   a.max_iter = max_iter;
-  a.xcd_map = 1;
-  a.out_iters = sb.iters;
-  a.out_rr = sb.rr;
-  a.out_status = sb.status;
-  TH2D_TRY(th2d_memset(ctx, sb.bar, 0, 16));
-  void *args[] = {&a};
   if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k0, ctx->stream));
-  hipError_t e = hipLaunchCooperativeKernel((const void *)k_cg<T>, dim3(nblocks), dim3(256), args,
-                                            lds, ctx->stream);
-  if (e != hipSuccess) {
-    fprintf(stderr, "th2d: cooperative launch of %d blocks failed: %s\n", nblocks,
-            hipGetErrorString(e));
-    return TH2D_ELAUNCH;
-  }
-  if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k1, ctx->stream));
-  // results: iterations, rr, status (one small copy; synchronises)
-  struct {
-    int iters, pad0;
-    double rr;
-    int status, pad1;
-  } h;
+  hipLaunchKernelGGL(k_cg_init<T>, dim3(nblocks), dim3(256), lds, ctx->stream, a);
+  TH2D_HIP(hipGetLastError());
   TH2D_TRY(th2d_host_reserve(ctx->redh, 64));
-  TH2D_HIP(hipMemcpyAsync(ctx->redh.ptr, sb.iters, 24, hipMemcpyDeviceToHost, ctx->stream));
-  TH2D_HIP(hipStreamSynchronize(ctx->stream));
-  memcpy(&h, ctx->redh.ptr, sizeof(h));
+  CgResult h;
+  memset(&h, 0, sizeof(h));
+  long launched = 0;
+  // first chunk: the iteration count of the previous solve of this
+  // precision (solves along a trajectory are alike), then steps of 32
+  int &last = ctx->cg_last_iters[sizeof(T) == 8 ? 1 : 0];
+  int chunk = ctx->cg_chunk > 0 ? ctx->cg_chunk : std::max(16, last + 4);
+  for (;;) {
+    for (int k = 0; k < chunk; k++) {
+      hipLaunchKernelGGL(k_cg_step<T>, dim3(nblocks), dim3(256), lds, ctx->stream, a,
+                         (int)((launched + k) & 1));
+    }
+    TH2D_HIP(hipGetLastError());
+    launched += chunk;
+    TH2D_HIP(hipMemcpyAsync(ctx->redh.ptr, sb.res, sizeof(CgResult), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    TH2D_HIP(hipStreamSynchronize(ctx->stream));
+    memcpy(&h, ctx->redh.ptr, sizeof(h));
+    if (h.done) break;
+    if (launched > (long)max_iter + 1) {  // cannot happen: the kernel stops at max_iter
+      h.status = TH2D_ENOCONV;
+      break;
+    }
+    chunk = ctx->cg_chunk > 0 ? ctx->cg_chunk : 32;
+  }
+  last = h.iters;
   if (ctx->ktiming) {
+    TH2D_HIP(hipEventRecord(ctx->k1, ctx->stream));
+    TH2D_HIP(hipEventSynchronize(ctx->k1));
     float ms = 0.f;
     TH2D_HIP(hipEventElapsedTime(&ms, ctx->k0, ctx->k1));
     ctx->ktime_ms += ms;
-    ctx->klaunches += 1;
+    ctx->klaunches += launched + 1;
     // algorithmic bytes of one CG iteration per site: two operator
     // applications (in + out + 2 links each: 8 complex) and the fused
     // vector updates (7 complex): 240 B in FP64, 120 B in FP32.
     double vb = sizeof(c2<T>);
     ctx->kbytes += (double)h.iters * V * 15.0 * vb;
+    ctx->kiters += h.iters;
   }
   res->iters = h.iters;
   res->rr = h.rr;
@@ -483,6 +487,13 @@ extern "C" int th2d_set_cg_max_blocks(th2d_ctx *ctx, int n)
 {
   if (!ctx || n < 0) return TH2D_EARG;
   ctx->cg_max_blocks = n;
+  return TH2D_OK;
+}
+
+extern "C" int th2d_set_cg_chunk(th2d_ctx *ctx, int n)
+{
+  if (!ctx || n < 0) return TH2D_EARG;
+  ctx->cg_chunk = n;
   return TH2D_OK;
 }
 

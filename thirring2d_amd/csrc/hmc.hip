@@ -52,6 +52,7 @@ int hmc_bufs(th2d_ctx *c, long V, HmcBufs &b)
   b.mchi = (double *)take(cv);
   b.stage = (double *)take(2 * cv);
   b.K32 = (float *)take(sizeof(float) * 4 * V);
+  if ((size_t)(p - (char *)c->hmc.ptr) > need) return TH2D_EARG;  // layout exceeds the block
   return TH2D_OK;
 }
 

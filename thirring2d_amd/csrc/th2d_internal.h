@@ -50,7 +50,10 @@ struct th2d_ctx {
   th2d_buf cg64, cg32, cgmix, red, redh, hmc, force, bag, misc;
   // tile configuration of the persistent solver (0 = autotuned default)
   int tile_t = 0, tile_x = 0;
-  int cg_max_blocks = 0;  // 0 = as many as fit co-resident
+  int cg_max_blocks = 0;  // 0 = automatic (4 per CU at most)
+  int cg_chunk = 0;       // step launches per stop-flag check (0 = 32, doubling)
+  long kiters = 0;
+  int cg_last_iters[2] = {0, 0};  // FP32, FP64 inner solves
 };
 
 int th2d_buf_reserve(th2d_buf &b, size_t bytes);
