@@ -1,0 +1,133 @@
+/*
+ * th2d_bag.h -- C ABI of the fermion-bag part of libth2d (gfx950):
+ * masked real staggered operator and its CG (vec_ops.c), dense batched LU,
+ * log-determinants and inverses (full_determinant.c, fermion_matrix.c),
+ * and the determinant-ratio state of the fermion-bag update
+ * (fluctuation_determinant.c).
+ *
+ * Conventions: the bag field is int32[V] in natural order s = t*nx + x with
+ * 0 = empty, 1 = monomer, 2+dir = link (Thirring.h:34-39); real vectors are
+ * double[V] in natural order.  Dense matrices are row-major.
+ */
+#ifndef TH2D_BAG_H
+#define TH2D_BAG_H
+
+#include "th2d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TH2D_BC_ANTISYMMETRIC 0 /* Thirring.h:27 */
+#define TH2D_BC_OPENX 1         /* Thirring.h:29, tests/thirring_brute.h:28 */
+
+/* Lattice of the fermion-bag family (Thirring.h:14-50, runtime values). */
+typedef struct {
+  int nt, nx;              /* even extents, >= 4                             */
+  int bc;                  /* TH2D_BC_*                                       */
+  int with_mass_monomers;  /* 1: no mass on the diagonal (Thirring.h:49)     */
+  double m, mu;
+} th2d_bag_geom;
+
+/* ---- masked operator and solver --------------------------------------- */
+/* out = fM in (transpose = 0) or fM^T in (vec_ops.c:96-172): rows of
+ * occupied sites are identity rows, hops into occupied sites are dropped.
+ * nrhs vectors are stored one after another (stride V). */
+int th2d_bag_apply(th2d_ctx *ctx, const th2d_bag_geom *g, const int *field, double *out,
+                   const double *in, int transpose, int nrhs);
+
+/* Solve fM x = src for nrhs right-hand sides by CG on fM^T fM after
+ * b = fM^T src (cg_propagator / cg_MdM, vec_ops.c:261-321), every RHS
+ * with its own alpha, beta and stop r.r < tol_abs_rr.  On divergence
+ * (r.r / r0.r0 > 1e10) that RHS is filled with 1e50 as the reference does.
+ * iters_host / status_host (nrhs each, may be NULL) receive per-RHS
+ * iteration counts and TH2D_OK / TH2D_ENOCONV / TH2D_EBREAKDOWN. */
+int th2d_bag_cg(th2d_ctx *ctx, const th2d_bag_geom *g, const int *field, double *x,
+                const double *src, int nrhs, double tol_abs_rr, int max_iter, int *iters_host,
+                int *status_host);
+
+/* ---- dense batched linear algebra ------------------------------------- */
+/* A batch of row-major square matrices in one device buffer: matrix b has
+ * order n[b], starts at element off[b]; its pivots start at poff[b].  The
+ * three arrays are device arrays of length count; max_n is host-side. */
+typedef struct {
+  int count;
+  int max_n;
+  const long *off;
+  const int *n;
+  const long *poff;
+} th2d_batch;
+
+/* In-place LU with partial pivoting (dgetrf semantics: row i swapped with
+ * piv[i] >= i, 0-based), FP64 or FP32 (TH2D_PREC_F64 / TH2D_PREC_F32); the
+ * trailing updates run on MFMA.  info[b] = 0 or j+1 at the first exactly
+ * zero pivot (device array). */
+int th2d_lu_batched(th2d_ctx *ctx, const th2d_batch *b, void *A, int precision, int *piv,
+                    int *info);
+/* log|det| and sign(det) from the factors (device arrays of length count);
+ * a zero pivot gives -inf and sign 0 (determinant_mu, full_determinant.c:
+ * 171-175, accumulated as logs). */
+int th2d_lu_logdet(th2d_ctx *ctx, const th2d_batch *b, const void *LU, int precision,
+                   const int *piv, const int *info, double *logabs, int *sign);
+/* X = A^{-1} from the factors (dgetri semantics); X uses the batch layout. */
+int th2d_lu_inverse(th2d_ctx *ctx, const th2d_batch *b, const void *LU, int precision,
+                    const int *piv, void *X);
+
+/* C += alpha A B for a batch of jobs (element offsets into one buffer). */
+typedef struct {
+  long a, b, c;
+  int lda, ldb, ldc;
+  int M, N, K;
+} th2d_gemm_job;
+int th2d_gemm_batched(th2d_ctx *ctx, void *base, int precision, const th2d_gemm_job *jobs_dev,
+                      int count, int max_m, int max_n, double alpha);
+/* Dense kernels on MFMA (1, default) or on the vector ALUs (0). */
+int th2d_set_dense_mfma(int on);
+int th2d_copy_f64_to_f32(th2d_ctx *ctx, float *dst, const double *src, long n);
+
+/* ---- matrices of bag configurations ---------------------------------- */
+/* Dense V x V masked matrix of `field` (identity rows and columns at
+ * occupied sites, fM_index elements between free sites; vec_ops.c:107-131,
+ * fermion_matrix.c:28-182) for a batch of configurations: config b has
+ * geometry geoms_dev[b], field at field_off[b], matrix at the batch's off[b]
+ * (FP64 or FP32). */
+int th2d_bag_build_masked(th2d_ctx *ctx, const th2d_bag_geom *geoms_dev, const int *fields,
+                          const long *field_off, const th2d_batch *b, void *A, int precision);
+
+/* ---- determinant-ratio state of the fermion-bag update ---------------- */
+/* Keeps B = (D_FF)^{-1}, the inverse of the fM_index matrix restricted to
+ * the free sites F of the accepted configuration (stored V x V with
+ * identity at occupied sites), and log|det D_FF| with its sign.  The
+ * reference's bag weight det_e det_o (fluctuation_determinant.c:203-237)
+ * equals det(D_FF)/det(D) by Jacobi's complementary-minor identity, so all
+ * of its determinant ratios follow from B:
+ *   occupy sites S:  ratio = det(B_SS)
+ *   free sites S:    ratio = det(D_SS - D_SF B D_FS)
+ * and an accepted change is a rank-|S| (Woodbury / Schur) update of B. */
+typedef struct th2d_bag th2d_bag;
+
+int th2d_bag_create(th2d_ctx *ctx, const th2d_bag_geom *g, th2d_bag **out);
+int th2d_bag_destroy(th2d_bag *bag);
+/* Load a configuration (host int32[V]) and refactor B from scratch by LU
+ * (update_background, fluctuation_determinant.c:126-301). */
+int th2d_bag_set_field(th2d_bag *bag, const int *field_host);
+int th2d_bag_refresh(th2d_bag *bag);
+/* log|det D_FF|, its sign, and log|det D| of the empty configuration. */
+int th2d_bag_logdet(th2d_bag *bag, double *logabs_ff, int *sign_ff, double *logabs_d, int *sign_d);
+/* Ratio det(D_F'F') / det(D_FF) for F' = F minus `occupy` plus `free_`
+ * (site indices, host arrays, at most 4 each); returns it in *ratio
+ * (signed). */
+int th2d_bag_ratio(th2d_bag *bag, const int *occupy, int n_occupy, const int *free_, int n_free,
+                   double *ratio);
+/* Apply the change of th2d_bag_ratio to B and to the log-determinant and
+ * set the field of the changed sites (values: field_vals for `occupy`,
+ * 0 for `free_`). */
+int th2d_bag_accept(th2d_bag *bag, const int *occupy, int n_occupy, const int *field_vals,
+                    const int *free_, int n_free);
+/* Device pointer of B and of the field (for tests and batched use). */
+int th2d_bag_views(th2d_bag *bag, double **B, int **field, int *V);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

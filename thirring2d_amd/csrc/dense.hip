@@ -1,0 +1,542 @@
+// dense.hip -- batched dense linear algebra on gfx950 for the fermion-bag
+// determinants (full_determinant.c:144-186, fluctuation_determinant.c:
+// 126-301, fermion_matrix.c:355-427): partial-pivot LU, log|det| and sign,
+// inverse from the LU factors, GEMM updates on the matrix cores.
+//
+// A batch is a set of row-major square matrices in one device buffer:
+// matrix b has order n[b], starts at element off[b], its pivots at poff[b].
+// Every kernel takes the whole batch; matrices of different orders share a
+// launch (blocks of a matrix that is already finished return at once).
+//
+// LU: right-looking, panel width NB = 32.  Per panel k:
+//   panel  -- one block per matrix factors columns k..k+NB-1 with partial
+//             pivoting (whole rows are swapped, as LAPACK's row interchanges
+//             do when applied to the full matrix);
+//   trsm   -- U12 = L11^{-1} A12 (unit lower, one thread per column);
+//   update -- A22 -= L21 U12, a GEMM on MFMA 16x16x4 tiles (FP64 or FP32).
+#include "th2d_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/th2d_bag.h"
+
+namespace {
+
+constexpr int NB = 32;    // LU panel width
+constexpr int BM = 64;    // GEMM tile rows
+constexpr int BN = 64;    // GEMM tile cols
+constexpr int BK = 16;    // GEMM k chunk
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  using acc = v4d;
+  static __device__ __forceinline__ acc op(double a, double b, acc c)
+  {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma<float> {
+  using acc = v4f;
+  static __device__ __forceinline__ acc op(float a, float b, acc c)
+  {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// C[i][j] += alpha * sum_k A[i][k] B[k][j] for one BM x BN tile at
+// (i0, j0); rows >= M, cols >= N, k >= K are skipped.  256 threads: wave w
+// owns the 32x32 quadrant (w/2, w%2) as 2x2 MFMA blocks of 16x16.
+// MFMA 16x16x4 operand layout (lane l): A[l%16][l/16], B[l/16][l%16],
+// result register r: D[4*(l/16)+r][l%16].
+template <typename T, bool USE_MFMA>
+__device__ void gemm_tile(const T *A, long lda, const T *B, long ldb, T *C, long ldc, int M, int N,
+                          int K, T alpha, int i0, int j0, T *sA, T *sB)
+{
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+  using Acc = typename Mfma<T>::acc;
+  Acc acc[2][2];
+  T vacc[4][4];
+  if (USE_MFMA) {
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) acc[a][b] = Acc{0, 0, 0, 0};
+  } else {
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) vacc[a][b] = 0;
+  }
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();
+    for (int e = tid; e < BM * BK; e += 256) {
+      int r = e / BK, c = e - r * BK;
+      int gi = i0 + r, gk = k0 + c;
+      sA[r * (BK + 1) + c] = (gi < M && gk < K) ? A[gi * lda + gk] : (T)0;
+    }
+    for (int e = tid; e < BK * BN; e += 256) {
+      int r = e / BN, c = e - r * BN;
+      int gk = k0 + r, gj = j0 + c;
+      sB[r * (BN + 1) + c] = (gk < K && gj < N) ? B[gk * ldb + gj] : (T)0;
+    }
+    __syncthreads();
+    if (USE_MFMA) {
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        T av[2], bv[2];
+        for (int a = 0; a < 2; a++) av[a] = sA[(wr + a * 16 + (lane & 15)) * (BK + 1) + kk + (lane >> 4)];
+        for (int b = 0; b < 2; b++) bv[b] = sB[(kk + (lane >> 4)) * (BN + 1) + wc + b * 16 + (lane & 15)];
+        for (int a = 0; a < 2; a++)
+          for (int b = 0; b < 2; b++) acc[a][b] = Mfma<T>::op(av[a], bv[b], acc[a][b]);
+      }
+    } else {
+      // thread owns the 4x4 block matching the MFMA result layout:
+      // rows wr + a*16 + 4*(lane/16) + r, cols wc + b*16 + lane%16
+      for (int kk = 0; kk < BK; kk++)
+        for (int a = 0; a < 2; a++)
+          for (int r = 0; r < 4; r++) {
+            T av = sA[(wr + a * 16 + 4 * (lane >> 4) + r) * (BK + 1) + kk];
+            for (int b = 0; b < 2; b++)
+              vacc[a * 2 + b][r] += av * sB[kk * (BN + 1) + wc + b * 16 + (lane & 15)];
+          }
+    }
+  }
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int r = 0; r < 4; r++) {
+        int gi = i0 + wr + a * 16 + 4 * (lane >> 4) + r;
+        int gj = j0 + wc + b * 16 + (lane & 15);
+        if (gi < M && gj < N) {
+          T v = USE_MFMA ? acc[a][b][r] : vacc[a * 2 + b][r];
+          C[gi * ldc + gj] += alpha * v;
+        }
+      }
+}
+
+struct BatchView {
+  const long *off;
+  const int *n;
+  const long *poff;
+  int count;
+};
+
+// ----------------------------------------------------------------- LU ---
+template <typename T>
+__global__ void __launch_bounds__(256) k_lu_panel(T *base, BatchView bv, int *pivb, int *info,
+                                                  int k)
+{
+  __shared__ double sval[256];
+  __shared__ int sidx[256];
+  const int b = blockIdx.x;
+  const int n = bv.n[b];
+  if (k >= n) return;
+  T *A = base + bv.off[b];
+  int *piv = pivb + bv.poff[b];
+  const int jb = min(NB, n - k), tid = threadIdx.x;
+  for (int j = k; j < k + jb; j++) {
+    // pivot: largest |a_ij|, i >= j; ties to the smaller row
+    double best = -1.0;
+    int bi = n;
+    for (int i = j + tid; i < n; i += 256) {
+      double v = fabs((double)A[(long)i * n + j]);
+      if (v > best) {
+        best = v;
+        bi = i;
+      }
+    }
+    sval[tid] = best;
+    sidx[tid] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) {
+        double v2 = sval[tid + s];
+        int i2 = sidx[tid + s];
+        if (v2 > sval[tid] || (v2 == sval[tid] && i2 < sidx[tid])) {
+          sval[tid] = v2;
+          sidx[tid] = i2;
+        }
+      }
+      __syncthreads();
+    }
+    const int p = sidx[0];
+    const double pabs = sval[0];
+    __syncthreads();
+    if (tid == 0) {
+      piv[j] = p;
+      if (pabs == 0.0 && info[b] == 0) info[b] = j + 1;
+    }
+    if (p != j)
+      for (int c = tid; c < n; c += 256) {
+        T t0 = A[(long)j * n + c];
+        A[(long)j * n + c] = A[(long)p * n + c];
+        A[(long)p * n + c] = t0;
+      }
+    __syncthreads();
+    const T pv = A[(long)j * n + j];
+    if (pv != (T)0)
+      for (int i = j + 1 + tid; i < n; i += 256) A[(long)i * n + j] /= pv;
+    __syncthreads();
+    const int w = k + jb - 1 - j;  // remaining panel columns
+    if (w > 0)
+      for (long e = tid; e < (long)(n - j - 1) * w; e += 256) {
+        int i = j + 1 + (int)(e / w), c = j + 1 + (int)(e % w);
+        A[(long)i * n + c] -= A[(long)i * n + j] * A[(long)j * n + c];
+      }
+    __syncthreads();
+  }
+}
+
+// U12 = L11^{-1} A12 on rows k..k+jb-1, columns c0.. (unit lower L11).
+template <typename T>
+__global__ void __launch_bounds__(256) k_lu_trsm(T *base, BatchView bv, int k)
+{
+  __shared__ T sL[NB * NB];
+  const int b = blockIdx.y;
+  const int n = bv.n[b];
+  if (k >= n) return;
+  const int jb = min(NB, n - k);
+  if (k + jb >= n) return;
+  T *A = base + bv.off[b];
+  for (int e = threadIdx.x; e < jb * jb; e += 256) {
+    int r = e / jb, q = e - r * jb;
+    sL[r * NB + q] = A[(long)(k + r) * n + k + q];
+  }
+  __syncthreads();
+  const int c = k + jb + blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  T x[NB];
+  for (int r = 0; r < jb; r++) x[r] = A[(long)(k + r) * n + c];
+  for (int r = 1; r < jb; r++) {
+    T s = x[r];
+    for (int q = 0; q < r; q++) s -= sL[r * NB + q] * x[q];
+    x[r] = s;
+  }
+  for (int r = 1; r < jb; r++) A[(long)(k + r) * n + c] = x[r];
+}
+
+// A22 -= L21 U12 for the trailing block after panel k.
+template <typename T, bool MF>
+__global__ void __launch_bounds__(256) k_lu_update(T *base, BatchView bv, int k)
+{
+  __shared__ T sA[BM * (BK + 1)];
+  __shared__ T sB[BK * (BN + 1)];
+  const int b = blockIdx.z;
+  const int n = bv.n[b];
+  const int jb = min(NB, n - k);
+  const int m = n - k - jb;
+  if (k >= n || m <= 0) return;
+  const int ti = blockIdx.y * BM, tj = blockIdx.x * BN;
+  if (ti >= m || tj >= m) return;
+  T *A = base + bv.off[b];
+  const long o = (long)(k + jb) * n + (k + jb);
+  gemm_tile<T, MF>(A + (long)(k + jb) * n + k, n, A + (long)k * n + (k + jb), n, A + o, n, m, m,
+                   jb, (T)-1, ti, tj, sA, sB);
+}
+
+// log|det| and sign from the factors: sign = prod sign(u_ii) * (-1)^{#swaps}
+template <typename T>
+__global__ void __launch_bounds__(256) k_lu_logdet(const T *base, BatchView bv, const int *pivb,
+                                                   const int *info, double *logabs, int *sign)
+{
+  __shared__ double sh[4];
+  __shared__ int shn[256];
+  const int b = blockIdx.x;
+  const int n = bv.n[b];
+  const T *A = base + bv.off[b];
+  const int *piv = pivb + bv.poff[b];
+  double acc = 0.0;
+  int neg = 0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    double u = (double)A[(long)i * n + i];
+    acc += log(fabs(u));
+    neg += (u < 0) + (piv[i] != i);
+  }
+  double s = block_sum(acc, sh);
+  shn[threadIdx.x] = neg;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int q = 0; q < 256; q++) t += shn[q];
+    if (info[b]) {
+      logabs[b] = -INFINITY;
+      sign[b] = 0;
+    } else {
+      logabs[b] = n ? s : 0.0;
+      sign[b] = (t & 1) ? -1 : 1;
+    }
+  }
+}
+
+// ------------------------------------------------------------ inverse ---
+// X = P (row permutation of the identity that the pivots define).
+template <typename T>
+__global__ void __launch_bounds__(256) k_perm_identity(T *X, BatchView bv, const int *pivb,
+                                                       int *permb)
+{
+  const int b = blockIdx.y;
+  const int n = bv.n[b];
+  const int *piv = pivb + bv.poff[b];
+  int *perm = permb + bv.poff[b];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int i = 0; i < n; i++) perm[i] = i;
+    for (int i = 0; i < n; i++) {
+      int p = piv[i];
+      int t0 = perm[i];
+      perm[i] = perm[p];
+      perm[p] = t0;
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill_perm(T *X, BatchView bv, const int *permb)
+{
+  const int b = blockIdx.y;
+  const int n = bv.n[b];
+  const int *perm = permb + bv.poff[b];
+  T *Xb = X + bv.off[b];
+  long total = (long)n * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    int i = (int)(e / n), c = (int)(e % n);
+    Xb[e] = (c == perm[i]) ? (T)1 : (T)0;
+  }
+}
+
+// Row block [k, k+jb) of X: X_blk = L11^{-1} X_blk (lower unit) or
+// U11^{-1} X_blk (upper), all n columns; one thread per column.
+template <typename T, bool UPPER>
+__global__ void __launch_bounds__(256) k_trsm_rows(const T *LU, T *X, BatchView bv, int k)
+{
+  __shared__ T sT[NB * NB];
+  const int b = blockIdx.y;
+  const int n = bv.n[b];
+  if (k >= n) return;
+  const int jb = min(NB, n - k);
+  const T *F = LU + bv.off[b];
+  T *Xb = X + bv.off[b];
+  for (int e = threadIdx.x; e < jb * jb; e += 256) {
+    int r = e / jb, q = e - r * jb;
+    sT[r * NB + q] = F[(long)(k + r) * n + k + q];
+  }
+  __syncthreads();
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  T x[NB];
+  for (int r = 0; r < jb; r++) x[r] = Xb[(long)(k + r) * n + c];
+  if (!UPPER) {
+    for (int r = 1; r < jb; r++) {
+      T s = x[r];
+      for (int q = 0; q < r; q++) s -= sT[r * NB + q] * x[q];
+      x[r] = s;
+    }
+  } else {
+    for (int r = jb - 1; r >= 0; r--) {
+      T s = x[r];
+      for (int q = r + 1; q < jb; q++) s -= sT[r * NB + q] * x[q];
+      x[r] = s / sT[r * NB + r];
+    }
+  }
+  for (int r = 0; r < jb; r++) Xb[(long)(k + r) * n + c] = x[r];
+}
+
+// Forward: X[k+jb:, :] -= L[k+jb:, k:k+jb] X[k:k+jb, :]
+// Backward: X[0:k, :] -= U[0:k, k:k+jb] X[k:k+jb, :]
+template <typename T, bool MF, bool BACK>
+__global__ void __launch_bounds__(256) k_inv_update(const T *LU, T *X, BatchView bv, int k)
+{
+  __shared__ T sA[BM * (BK + 1)];
+  __shared__ T sB[BK * (BN + 1)];
+  const int b = blockIdx.z;
+  const int n = bv.n[b];
+  if (k >= n) return;
+  const int jb = min(NB, n - k);
+  const int m = BACK ? k : n - k - jb;
+  if (m <= 0) return;
+  const int ti = blockIdx.y * BM, tj = blockIdx.x * BN;
+  if (ti >= m || tj >= n) return;
+  const T *F = LU + bv.off[b];
+  T *Xb = X + bv.off[b];
+  const int r0 = BACK ? 0 : k + jb;
+  gemm_tile<T, MF>(F + (long)r0 * n + k, n, Xb + (long)k * n, n, Xb + (long)r0 * n, n, m, n, jb,
+                   (T)-1, ti, tj, sA, sB);
+}
+
+// generic batched GEMM: C += alpha A B (jobs in device memory)
+template <typename T, bool MF>
+__global__ void __launch_bounds__(256) k_gemm_jobs(T *base, const th2d_gemm_job *jobs, double alpha)
+{
+  __shared__ T sA[BM * (BK + 1)];
+  __shared__ T sB[BK * (BN + 1)];
+  const th2d_gemm_job J = jobs[blockIdx.z];
+  const int ti = blockIdx.y * BM, tj = blockIdx.x * BN;
+  if (ti >= J.M || tj >= J.N) return;
+  gemm_tile<T, MF>(base + J.a, J.lda, base + J.b, J.ldb, base + J.c, J.ldc, J.M, J.N, J.K,
+                   (T)alpha, ti, tj, sA, sB);
+}
+
+__global__ void k_f64_to_f32_n(float *d, const double *s, long n)
+{
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+    d[i] = (float)s[i];
+}
+
+static int g_use_mfma = -1;
+bool use_mfma()
+{
+  if (g_use_mfma < 0) {
+    const char *e = getenv("TH2D_DENSE_MFMA");
+    g_use_mfma = e ? atoi(e) != 0 : 1;
+  }
+  return g_use_mfma != 0;
+}
+
+BatchView view(const th2d_batch *b) { return BatchView{b->off, b->n, b->poff, b->count}; }
+
+template <typename T>
+int lu_run(th2d_ctx *c, const th2d_batch *bt, T *A, int *piv, int *info)
+{
+  BatchView bv = view(bt);
+  TH2D_TRY(th2d_memset(c, info, 0, sizeof(int) * bt->count));
+  const bool mf = use_mfma();
+  for (int k = 0; k < bt->max_n; k += NB) {
+    hipLaunchKernelGGL(k_lu_panel<T>, dim3(bt->count), dim3(256), 0, c->stream, A, bv, piv, info,
+                       k);
+    int rest = bt->max_n - k - NB;
+    if (rest > 0) {
+      hipLaunchKernelGGL(k_lu_trsm<T>, dim3((rest + 255) / 256, bt->count), dim3(256), 0,
+                         c->stream, A, bv, k);
+      dim3 g((rest + BN - 1) / BN, (rest + BM - 1) / BM, bt->count);
+      if (mf) hipLaunchKernelGGL((k_lu_update<T, true>), g, dim3(256), 0, c->stream, A, bv, k);
+      else hipLaunchKernelGGL((k_lu_update<T, false>), g, dim3(256), 0, c->stream, A, bv, k);
+    }
+  }
+  TH2D_HIP(hipGetLastError());
+  return TH2D_OK;
+}
+
+template <typename T>
+int inverse_run(th2d_ctx *c, const th2d_batch *bt, const T *LU, const int *piv, T *X)
+{
+  BatchView bv = view(bt);
+  const int N = bt->max_n;
+  const bool mf = use_mfma();
+  size_t need = 0;
+  // scratch: permutation per matrix (same offsets as the pivots)
+  {
+    long total = 0;
+    std::vector<long> h(bt->count);
+    TH2D_HIP(hipMemcpyAsync(h.data(), bt->poff, sizeof(long) * bt->count, hipMemcpyDeviceToHost,
+                            c->stream));
+    TH2D_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < bt->count; i++) total = std::max(total, h[i]);
+    need = sizeof(int) * (size_t)(total + N + 64);
+  }
+  TH2D_TRY(th2d_buf_reserve(c->bag, need));
+  int *perm = (int *)c->bag.ptr;
+  hipLaunchKernelGGL(k_perm_identity<T>, dim3(1, bt->count), dim3(256), 0, c->stream, X, bv, piv,
+                     perm);
+  hipLaunchKernelGGL(k_fill_perm<T>, dim3(std::min(1024, (N * N + 255) / 256), bt->count),
+                     dim3(256), 0, c->stream, X, bv, perm);
+  // forward substitution with L (unit lower)
+  for (int k = 0; k < N; k += NB) {
+    hipLaunchKernelGGL((k_trsm_rows<T, false>), dim3((N + 255) / 256, bt->count), dim3(256), 0,
+                       c->stream, LU, X, bv, k);
+    int rest = N - k - NB;
+    if (rest > 0) {
+      dim3 g((N + BN - 1) / BN, (rest + BM - 1) / BM, bt->count);
+      if (mf) hipLaunchKernelGGL((k_inv_update<T, true, false>), g, dim3(256), 0, c->stream, LU,
+                                 X, bv, k);
+      else hipLaunchKernelGGL((k_inv_update<T, false, false>), g, dim3(256), 0, c->stream, LU, X,
+                              bv, k);
+    }
+  }
+  // back substitution with U: the last block first
+  int klast = ((N - 1) / NB) * NB;
+  for (int k = klast; k >= 0; k -= NB) {
+    hipLaunchKernelGGL((k_trsm_rows<T, true>), dim3((N + 255) / 256, bt->count), dim3(256), 0,
+                       c->stream, LU, X, bv, k);
+    if (k > 0) {
+      dim3 g((N + BN - 1) / BN, (k + BM - 1) / BM, bt->count);
+      if (mf) hipLaunchKernelGGL((k_inv_update<T, true, true>), g, dim3(256), 0, c->stream, LU, X,
+                                 bv, k);
+      else hipLaunchKernelGGL((k_inv_update<T, false, true>), g, dim3(256), 0, c->stream, LU, X,
+                              bv, k);
+    }
+  }
+  TH2D_HIP(hipGetLastError());
+  return TH2D_OK;
+}
+
+}  // namespace
+
+// ================================================================ ABI ===
+extern "C" int th2d_lu_batched(th2d_ctx *c, const th2d_batch *b, void *A, int precision, int *piv,
+                               int *info)
+{
+  if (!c || !b || !A || !piv || !info || b->count < 1 || b->max_n < 0) return TH2D_EARG;
+  if (precision == TH2D_PREC_F64) return lu_run<double>(c, b, (double *)A, piv, info);
+  if (precision == TH2D_PREC_F32) return lu_run<float>(c, b, (float *)A, piv, info);
+  return TH2D_EARG;
+}
+
+extern "C" int th2d_lu_logdet(th2d_ctx *c, const th2d_batch *b, const void *LU, int precision,
+                              const int *piv, const int *info, double *logabs, int *sign)
+{
+  if (!c || !b || !LU || !piv || !info || !logabs || !sign || b->count < 1) return TH2D_EARG;
+  BatchView bv = view(b);
+  if (precision == TH2D_PREC_F64)
+    hipLaunchKernelGGL(k_lu_logdet<double>, dim3(b->count), dim3(256), 0, c->stream,
+                       (const double *)LU, bv, piv, info, logabs, sign);
+  else if (precision == TH2D_PREC_F32)
+    hipLaunchKernelGGL(k_lu_logdet<float>, dim3(b->count), dim3(256), 0, c->stream,
+                       (const float *)LU, bv, piv, info, logabs, sign);
+  else return TH2D_EARG;
+  TH2D_HIP(hipGetLastError());
+  return TH2D_OK;
+}
+
+extern "C" int th2d_lu_inverse(th2d_ctx *c, const th2d_batch *b, const void *LU, int precision,
+                               const int *piv, void *X)
+{
+  if (!c || !b || !LU || !piv || !X || LU == X || b->count < 1) return TH2D_EARG;
+  if (precision == TH2D_PREC_F64) return inverse_run<double>(c, b, (const double *)LU, piv, (double *)X);
+  if (precision == TH2D_PREC_F32) return inverse_run<float>(c, b, (const float *)LU, piv, (float *)X);
+  return TH2D_EARG;
+}
+
+extern "C" int th2d_gemm_batched(th2d_ctx *c, void *base, int precision,
+                                 const th2d_gemm_job *jobs_dev, int count, int max_m, int max_n,
+                                 double alpha)
+{
+  if (!c || !base || !jobs_dev || count < 1 || max_m < 0 || max_n < 0) return TH2D_EARG;
+  if (!max_m || !max_n) return TH2D_OK;
+  dim3 g((max_n + BN - 1) / BN, (max_m + BM - 1) / BM, count);
+  const bool mf = use_mfma();
+  if (precision == TH2D_PREC_F64) {
+    if (mf) hipLaunchKernelGGL((k_gemm_jobs<double, true>), g, dim3(256), 0, c->stream, (double *)base, jobs_dev, alpha);
+    else hipLaunchKernelGGL((k_gemm_jobs<double, false>), g, dim3(256), 0, c->stream, (double *)base, jobs_dev, alpha);
+  } else if (precision == TH2D_PREC_F32) {
+    if (mf) hipLaunchKernelGGL((k_gemm_jobs<float, true>), g, dim3(256), 0, c->stream, (float *)base, jobs_dev, alpha);
+    else hipLaunchKernelGGL((k_gemm_jobs<float, false>), g, dim3(256), 0, c->stream, (float *)base, jobs_dev, alpha);
+  } else return TH2D_EARG;
+  TH2D_HIP(hipGetLastError());
+  return TH2D_OK;
+}
+
+extern "C" int th2d_set_dense_mfma(int on)
+{
+  g_use_mfma = on ? 1 : 0;
+  return TH2D_OK;
+}
+
+extern "C" int th2d_copy_f64_to_f32(th2d_ctx *c, float *d, const double *s, long n)
+{
+  if (!c || (!d && n) || (!s && n) || n < 0) return TH2D_EARG;
+  if (!n) return TH2D_OK;
+  long blocks = std::min(4096L, (n + 255) / 256);
+  hipLaunchKernelGGL(k_f64_to_f32_n, dim3((unsigned)blocks), dim3(256), 0, c->stream, d, s, n);
+  TH2D_HIP(hipGetLastError());
+  return TH2D_OK;
+}
