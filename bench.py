@@ -7,10 +7,12 @@ steps of dt = 0.1 per trajectory, pseudofermion / momentum / stochastic
 noise from the device generator, every CG solve to a true relative
 residual of 1e-10.  One step = one trajectory.
 
-Prints one JSON line.  `value` is trajectories per second of the
-mixed-precision path (FP32 inner CG, FP64 reliable updates); the FP64 path
-is timed beside it.  `roofline` is for the dominant kernel, the persistent
-CG solver kernel, timed with HIP events on its own stream.
+Prints one JSON line.  `value` is the aggregate trajectories per second of
+a batch of independent 256^2 chains advanced in lockstep (every inversion
+of all chains is one batched solve) with mixed precision (FP32 CG, FP64
+true-residual restarts); the batched FP64 path and single chains in FP64
+and mixed precision are timed beside it.  `roofline` is for the dominant
+kernel, the CG step kernel, timed with HIP events on its stream.
 """
 from __future__ import annotations
 
@@ -34,36 +36,45 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--chains", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dslash", action="store_true")
     return ap.parse_args()
 
 
-def run_hmc(H, N, size, precision, steps, warmup, seed_links=1002):
-    lat = H.StaggeredLattice(size, size, m=0.05, g=0.5)
-    lat.random_links(seed_links)
+def run_hmc(H, N, size, precision, steps, warmup, chains=1, seed_links=1002):
+    """Time `steps` trajectories of `chains` independent chains (one batched
+    call per step).  Returns (seconds, per-trajectory records, kernel ms,
+    kernel launches, kernel bytes)."""
+    import ctypes as C
+    lats = []
+    for c in range(chains):
+        lat = H.StaggeredLattice(size, size, m=0.05, g=0.5)
+        lat.random_links(seed_links + c)
+        lats.append(lat)
+    batch = H.ChainBatch(lats)
     params = H.cg_params(tol_rel=1e-10, precision=precision, max_iter=200000)
     L = N.lib()
     ctr = 0
     for _ in range(warmup):
-        lat.update_gauge(nsteps=10, traj_len=1.0, params=params, seed=4242, counter=ctr)
+        batch.update_gauge(nsteps=10, traj_len=1.0, params=params, seed=4242,
+                           counters=[ctr] * chains)
         ctr += 1
-    L.th2d_kernel_timing(lat.ctx.ptr, 1)
-    lat.ctx.sync()
+    L.th2d_kernel_timing(batch.ctx.ptr, 1)
+    batch.ctx.sync()
     t0 = time.perf_counter()
     res = []
     for _ in range(steps):
-        r = lat.update_gauge(nsteps=10, traj_len=1.0, params=params, seed=4242, counter=ctr)
+        rr = batch.update_gauge(nsteps=10, traj_len=1.0, params=params, seed=4242,
+                                counters=[ctr] * chains, u_accept=list(np.random.default_rng(ctr).random(chains)))
         ctr += 1
-        res.append((r.dS, r.accepted, r.cg_iterations, r.cg_solves, r.max_true_rel_resid,
-                    r.status))
-    lat.ctx.sync()
+        res += [(r.dS, r.accepted, r.cg_iterations, r.cg_solves, r.max_true_rel_resid, r.status)
+                for r in rr]
+    batch.ctx.sync()
     dt = time.perf_counter() - t0
-    import ctypes as C
     ms, n, by = C.c_double(), C.c_long(), C.c_double()
-    L.th2d_kernel_time(lat.ctx.ptr, C.byref(ms), C.byref(n), C.byref(by))
-    L.th2d_kernel_timing(lat.ctx.ptr, 0)
+    L.th2d_kernel_time(batch.ctx.ptr, C.byref(ms), C.byref(n), C.byref(by))
+    L.th2d_kernel_timing(batch.ctx.ptr, 0)
     return dt, res, ms.value, n.value, by.value
 
 
@@ -121,19 +132,24 @@ def main():
     from thirring2d_amd import _native as N
     from thirring2d_amd import hmc as H
 
-    size = args.size
-    dt_mx, res_mx, kms, kn, kbytes = run_hmc(H, N, size, N.PREC_MIXED, args.steps, args.warmup)
-    fp64_steps = args.fp64_steps if args.fp64_steps is not None else max(2, args.steps // 4)
-    dt_64, res_64, kms64, kn64, kbytes64 = run_hmc(H, N, size, N.PREC_F64, fp64_steps, 1)
-    tps_mx = args.steps / dt_mx
-    tps_64 = fp64_steps / dt_64
-    its_mx = float(np.mean([r[2] for r in res_mx]))
-    its_64 = float(np.mean([r[2] for r in res_64]))
+    size, nch = args.size, args.chains
+    # headline: batched mixed precision
+    dt_mx, res_mx, kms, kn, kbytes = run_hmc(H, N, size, N.PREC_MIXED, args.steps, args.warmup,
+                                             chains=nch)
+    small = max(2, args.steps // 4)
+    dt_b64, res_b64, *_ = run_hmc(H, N, size, N.PREC_F64, small, 1, chains=nch)
+    dt_1mx, res_1mx, *_ = run_hmc(H, N, size, N.PREC_MIXED, small, 1, chains=1)
+    dt_164, res_164, *_ = run_hmc(H, N, size, N.PREC_F64, small, 1, chains=1)
+    tps = nch * args.steps / dt_mx
+    tps_b64 = nch * small / dt_b64
+    tps_1mx = small / dt_1mx
+    tps_164 = small / dt_164
+    its_164 = float(np.mean([r[2] for r in res_164]))
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
 
     out = {
-        "metric": "HMC traj/s at 256^2 (mixed precision, CG true residual <= 1e-10)",
-        "value": tps_mx,
+        "metric": "HMC traj/s at 256^2 (mixed precision, batched chains, CG true residual <= 1e-10)",
+        "value": tps,
         "unit": "traj/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -143,20 +159,22 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
-        "data": "synthetic: A~N(0,1) seed 1002, device Philox noise",
-        "config": {"workload": f"HMC {size}x{size}, m=0.05, g=0.5, 10 x dt=0.1, CG 1e-10",
-                   "lattice": [size, size], "nsteps": 10, "dt": 0.1, "tol_rel": 1e-10},
-        "fp64": {"value": tps_64, "unit": "traj/s", "steps": fp64_steps,
-                 "cg_iters_per_traj": its_64,
-                 "max_true_rel_resid": max(r[4] for r in res_64)},
-        "mixed_over_fp64": tps_mx / tps_64,
-        "cg_iters_per_traj": its_mx,
-        "max_true_rel_resid": max(r[4] for r in res_mx),
+        "data": "synthetic: A~N(0,1) seeds 1002+chain, device Philox noise",
+        "config": {"workload": f"HMC {size}x{size}, m=0.05, g=0.5, 10 x dt=0.1, CG 1e-10, "
+                               f"{nch} chains per batched step",
+                   "lattice": [size, size], "chains": nch, "nsteps": 10, "dt": 0.1,
+                   "tol_rel": 1e-10},
+        "batched_fp64": {"value": tps_b64, "unit": "traj/s", "chains": nch},
+        "single_chain_mixed": {"value": tps_1mx, "unit": "traj/s"},
+        "single_chain_fp64": {"value": tps_164, "unit": "traj/s", "cg_iters_per_traj": its_164},
+        "batched_mixed_over_single_fp64": tps / tps_164,
+        "cg_iters_per_traj": float(np.mean([r[2] for r in res_mx])),
+        "max_true_rel_resid": max(r[4] for r in res_mx + res_b64 + res_1mx + res_164),
         "acceptance": float(np.mean([r[1] for r in res_mx])),
         "dH_median": float(np.median([r[0] for r in res_mx])),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_cg<float> (persistent CG, 120 B/site/iteration)",
+                     "kernel": "k_cg_step<float> (CG iteration, 120 B/site algorithmic)",
                      "launches": kn, "avg_launch_ms": kms / max(kn, 1)},
     }
     if not args.no_dslash:
@@ -169,7 +187,7 @@ def main():
             d[tag] = {"ms": ms, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS}
         out["dslash"] = d
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(its_64, size)
+        out["cpu_baseline"] = cpu_baseline(its_164, size)
     print(json.dumps(out), flush=True)
 
 

@@ -139,12 +139,25 @@ int th2d_cg_mdagm(th2d_ctx *ctx, const th2d_geom *g, double *x, const double *b,
                   const double *K, const float *K32, const th2d_cg_params *p,
                   th2d_cg_info *info);
 
-/* Tuning of the persistent solver and of the operator kernel: tile shape
- * (tt x tx sites per workgroup tile, 0 0 = automatic), a cap on the number
- * of co-resident workgroups (0 = as many as fit), and the Dslash variant
- * (0 direct, 1 LDS-tiled). */
+/* The same solve for nch independent lattices of any sizes in one batch:
+ * every launch of the solver kernel serves all of them (per-lattice
+ * scalars and stop decisions).  geoms, x, b, K, K32 and infos are host
+ * arrays of length nch (x, b, K, K32 hold device pointers).  Returns the
+ * first non-OK status, if any. */
+int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *geoms, double **x,
+                          const double *const *b, const double *const *K,
+                          const float *const *K32, const th2d_cg_params *p,
+                          th2d_cg_info *infos);
+
+/* Tuning of the solver and of the operator kernel: tile shape (tt x tx
+ * sites per workgroup tile, (tt+4)(tx+4) <= 1024; 0 0 = automatic), a cap
+ * on the number of workgroups of a solver launch (0 = 8 per CU), and the
+ * Dslash variant (0 direct, 1 LDS-tiled). */
 int th2d_set_cg_tile(th2d_ctx *ctx, int tt, int tx);
 int th2d_set_cg_max_blocks(th2d_ctx *ctx, int n);
+/* Solver launches enqueued between two reads of the stop flags
+ * (0 = the previous solve's iteration count, then 32). */
+int th2d_set_cg_chunk(th2d_ctx *ctx, int n);
 int th2d_set_dslash_variant(int variant);
 
 /* ---- force -------------------------------------------------------------- */
@@ -195,6 +208,19 @@ int th2d_hmc_trajectory(th2d_ctx *ctx, const th2d_geom *geo, const th2d_hmc_para
                         double *A, const double *xi_host, const double *mom_host,
                         const double *st_host, double u_accept, unsigned long long seed,
                         unsigned long long counter, th2d_hmc_result *res);
+
+/* nch independent chains, one trajectory each, in lockstep: every
+ * inversion of all chains is one batched solve.  geos, hps (one per chain:
+ * g and nf may differ, nsteps/traj_len/phys_norm/cg must agree), A, u_accept
+ * and res are host arrays of length nch (A holds device pointers).  xi_h,
+ * mom_h, st_h are NULL (device generator: chain k uses streams 4k+1..4k+3 of
+ * `seed` at counters[k]) or host arrays of host noise pointers. */
+int th2d_hmc_trajectory_batched(th2d_ctx *ctx, int nch, const th2d_geom *geos,
+                                const th2d_hmc_params *hps, double **A,
+                                const double *const *xi_h, const double *const *mom_h,
+                                const double *const *st_h, const double *u_accept,
+                                unsigned long long seed, const unsigned long long *counters,
+                                th2d_hmc_result *res);
 
 /* Gaussian N(0,1) link angles from the device generator (configs: A~N(0,1)). */
 int th2d_random_links(th2d_ctx *ctx, const th2d_geom *g, double *A, unsigned long long seed);

@@ -127,6 +127,19 @@ int th2d_bag_accept(th2d_bag *bag, const int *occupy, int n_occupy, const int *f
 /* Device pointer of B and of the field (for tests and batched use). */
 int th2d_bag_views(th2d_bag *bag, double **B, int **field, int *V);
 
+/* Free propagator D^{-1} in the reference's eo block layout
+ * (calc_Dinv, fermion_matrix.c:191-351): Dinv[e*V/2 + o] = (D^{-1})_{e,o},
+ * Dinv[V^2/4 + o*V/2 + e] = (D^{-1})_{o,e} (device, 2 (V/2)^2 doubles),
+ * computed by dense LU inversion on the device. */
+int th2d_bag_calc_dinv(th2d_ctx *ctx, const th2d_bag_geom *g, double *Dinv);
+
+/* Bag weights det_e det_o (fluctuation_determinant.c:203-237,
+ * tests/thirring_brute.c:273-345) of `count` configurations given as uint8
+ * fields (natural order, device) from Dinv; w[c] (device) is 0 for
+ * configurations the reference gives weight 0.  V/2 <= 16. */
+int th2d_bag_small_weights(th2d_ctx *ctx, const th2d_bag_geom *g, const double *Dinv,
+                           const unsigned char *cfg, long count, double *w);
+
 #ifdef __cplusplus
 }
 #endif

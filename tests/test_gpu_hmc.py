@@ -247,3 +247,55 @@ def test_trajectory_mixed_vs_fp64(hmc):
         out.append(res.dS)
     assert abs(out[0]) < 10.0
     assert abs(out[0] - out[1]) <= 1e-5 + 1e-6 * abs(out[0])
+
+
+def test_batched_cg_uneven(hmc, oracle):
+    """One batched solve of lattices of different sizes and masses: every
+    chain reaches its own true residual (FP64 and mixed)."""
+    from thirring2d_amd import _native as N
+    dims = [(16, 16), (8, 12), (24, 16), (32, 32)]
+    setups = [_setup(hmc, oracle, nt, nx, 100 + k, m=0.1 + 0.05 * k)
+              for k, (nt, nx) in enumerate(dims)]
+    batch = hmc.ChainBatch([s[0] for s in setups])
+    for prec, tol in ((N.PREC_F64, 1e-12), (N.PREC_MIXED, 1e-10)):
+        xs, infos = batch.fmdm_invert_cg([s[3] for s in setups],
+                                         hmc.cg_params(tol_rel=tol, precision=prec))
+        for (lat, L, A, b, _), x, inf in zip(setups, xs, infos):
+            assert inf.status == 0
+            r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
+            assert np.linalg.norm(r) / np.linalg.norm(b) <= tol
+
+
+def test_batched_trajectory_matches_oracle(hmc, oracle):
+    """Chains of different sizes and couplings in one batched trajectory,
+    each with its own MT noise: every chain equals the oracle's trajectory
+    (stable step, 20 x dt = 0.05)."""
+    dims = [(16, 16), (8, 12), (12, 8)]
+    gs = [0.5, 0.8, 0.3]
+    lats, Ls, A0s, noise, us = [], [], [], [], []
+    orc_rngs = []
+    for k, ((nt, nx), g) in enumerate(zip(dims, gs)):
+        V = nt * nx
+        A0 = np.random.default_rng(300 + k).normal(size=(V, 2))
+        lat = hmc.StaggeredLattice(nt, nx, m=0.2, g=g)
+        lat.set_links(A0)
+        lats.append(lat)
+        Ls.append(oracle.lattice(nt, nx, m=0.2, g=g))
+        A0s.append(A0)
+        r = oracle.Rng(5000 + k, warmup=100)
+        xi = np.array([complex(*_pair(r)) for _ in range(V)])
+        mom = np.array([_pair(r) for _ in range(V)])
+        st = np.array([complex(*_pair(r)) for _ in range(V)])
+        us.append(r.next())
+        noise.append((xi, mom, st))
+        orc_rngs.append(oracle.Rng(5000 + k, warmup=100))
+    batch = hmc.ChainBatch(lats)
+    res = batch.update_gauge(nsteps=20, traj_len=1.0, noise=noise, u_accept=us,
+                             params=hmc.cg_params(tol_rel=1e-12))
+    for k in range(3):
+        acc, dS, acts, its, A1 = oracle.update_gauge(Ls[k], orc_rngs[k], A0s[k], nsteps=20,
+                                                     tol_rel=1e-12)
+        assert res[k].status == 0
+        assert abs(res[k].dS - dS) <= 1e-8 * max(1.0, abs(dS))
+        assert res[k].accepted == acc
+        np.testing.assert_allclose(lats[k].get_links(), A1, rtol=0, atol=1e-9)

@@ -62,6 +62,15 @@ class BagGeom(C.Structure):
                 ("mu", D)]
 
 
+class Batch(C.Structure):
+    _fields_ = [("count", I), ("max_n", I), ("off", P), ("n", P), ("poff", P)]
+
+
+class GemmJob(C.Structure):
+    _fields_ = [("a", L), ("b", L), ("c", L), ("lda", I), ("ldb", I), ("ldc", I),
+                ("M", I), ("N", I), ("K", I)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/*.h
 SIGNATURES = {
     # th2d.h
@@ -104,6 +113,34 @@ SIGNATURES = {
     "th2d_set_cg_tile": (I, [P, I, I]),
     "th2d_set_dslash_variant": (I, [I]),
     "th2d_set_cg_max_blocks": (I, [P, I]),
+    "th2d_set_cg_chunk": (I, [P, I]),
+    "th2d_cg_mdagm_batched": (I, [P, I, C.POINTER(Geom), C.POINTER(P), C.POINTER(P),
+                                  C.POINTER(P), C.POINTER(P), C.POINTER(CgParams),
+                                  C.POINTER(CgInfo)]),
+    "th2d_hmc_trajectory_batched": (I, [P, I, C.POINTER(Geom), C.POINTER(HmcParams),
+                                        C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P),
+                                        C.POINTER(D), U64, C.POINTER(U64),
+                                        C.POINTER(HmcResult)]),
+    # th2d_bag.h
+    "th2d_bag_apply": (I, [P, C.POINTER(BagGeom), P, P, P, I, I]),
+    "th2d_bag_cg": (I, [P, C.POINTER(BagGeom), P, P, P, I, D, I, C.POINTER(I), C.POINTER(I)]),
+    "th2d_lu_batched": (I, [P, C.POINTER(Batch), P, I, P, P]),
+    "th2d_lu_logdet": (I, [P, C.POINTER(Batch), P, I, P, P, P, P]),
+    "th2d_lu_inverse": (I, [P, C.POINTER(Batch), P, I, P, P]),
+    "th2d_gemm_batched": (I, [P, P, I, P, I, I, I, D]),
+    "th2d_set_dense_mfma": (I, [I]),
+    "th2d_copy_f64_to_f32": (I, [P, P, P, L]),
+    "th2d_bag_build_masked": (I, [P, P, P, P, C.POINTER(Batch), P, I]),
+    "th2d_bag_create": (I, [P, C.POINTER(BagGeom), C.POINTER(P)]),
+    "th2d_bag_destroy": (I, [P]),
+    "th2d_bag_set_field": (I, [P, P]),
+    "th2d_bag_refresh": (I, [P]),
+    "th2d_bag_logdet": (I, [P, C.POINTER(D), C.POINTER(I), C.POINTER(D), C.POINTER(I)]),
+    "th2d_bag_ratio": (I, [P, P, I, P, I, C.POINTER(D)]),
+    "th2d_bag_accept": (I, [P, P, I, P, P, I]),
+    "th2d_bag_views": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(I)]),
+    "th2d_bag_calc_dinv": (I, [P, C.POINTER(BagGeom), P]),
+    "th2d_bag_small_weights": (I, [P, C.POINTER(BagGeom), P, P, L, P]),
 }
 
 _lib = None

@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(256) k_reduce2(int nparts, const double *part,
 
 namespace th2d {
 
-static int reduce_op(th2d_ctx *c, int op, long n, const double *a, const double *b, double *dout,
+int reduce_op(th2d_ctx *c, int op, long n, const double *a, const double *b, double *dout,
                      double scale)
 {
   TH2D_TRY(th2d_buf_reserve(c->red, sizeof(double) * (RED_BLOCKS + 64)));

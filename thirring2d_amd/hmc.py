@@ -242,3 +242,69 @@ class StaggeredLattice:
             N.check(rc, "th2d_hmc_trajectory")
         self.refold()
         return res
+
+
+class ChainBatch:
+    """Independent chains advanced in lockstep: every inversion of all
+    chains is one batched solve (th2d_hmc_trajectory_batched).  Chains may
+    have different lattice sizes, couplings and masses."""
+
+    def __init__(self, lattices):
+        self.lats = list(lattices)
+        self.ctx = Context.get()
+        n = len(self.lats)
+        self._geoms = (N.Geom * n)(*[l.geom for l in self.lats])
+
+    def __len__(self):
+        return len(self.lats)
+
+    def update_gauge(self, nsteps=10, traj_len=1.0, params=None, phys_norm=0, noise=None,
+                     u_accept=None, seed=0, counters=None):
+        """One trajectory per chain; noise = list of (xi, mom, st) or None."""
+        n = len(self.lats)
+        params = params or cg_params()
+        hps = (N.HmcParams * n)(*[N.HmcParams(l.g, l.nf, nsteps, traj_len, phys_norm, params)
+                                  for l in self.lats])
+        As = (C.c_void_p * n)(*[l.A.p for l in self.lats])
+        u = (C.c_double * n)(*(u_accept if u_accept is not None else [0.5] * n))
+        ctr = (C.c_ulonglong * n)(*(counters if counters is not None else [0] * n))
+        res = (N.HmcResult * n)()
+        keep = []
+        if noise is not None:
+            ptrs = [[], [], []]
+            for (xi, mom, st) in noise:
+                arrs = (np.ascontiguousarray(xi, dtype=np.complex128),
+                        np.ascontiguousarray(mom, dtype=np.float64),
+                        np.ascontiguousarray(st, dtype=np.complex128))
+                keep.append(arrs)
+                for q in range(3):
+                    ptrs[q].append(arrs[q].ctypes.data_as(C.c_void_p))
+            xs, ms, ss = ((C.c_void_p * n)(*p) for p in ptrs)
+        else:
+            xs = ms = ss = None
+        rc = N.lib().th2d_hmc_trajectory_batched(self.ctx.ptr, n, self._geoms, hps, As, xs, ms,
+                                                 ss, u, seed, ctr, res)
+        if rc not in (N.TH2D_OK, N.TH2D_ENOCONV, N.TH2D_EBREAKDOWN):
+            N.check(rc, "th2d_hmc_trajectory_batched")
+        for l in self.lats:
+            l.refold()
+        return list(res)
+
+    def fmdm_invert_cg(self, bs, params=None):
+        """Batched x_k = (M_k^dag M_k)^{-1} b_k; returns ([x_k], [CgInfo])."""
+        n = len(self.lats)
+        params = params or cg_params()
+        xs_d = [DeviceArray(16 * l.V) for l in self.lats]
+        bs_d = [DeviceArray(16 * l.V) for l in self.lats]
+        for l, b, bd in zip(self.lats, bs, bs_d):
+            l.to_eo(b, bd)
+        xs = (C.c_void_p * n)(*[d.p for d in xs_d])
+        bp = (C.c_void_p * n)(*[d.p for d in bs_d])
+        ks = (C.c_void_p * n)(*[l.K.p for l in self.lats])
+        k32 = (C.c_void_p * n)(*[l.K32.p for l in self.lats])
+        infos = (N.CgInfo * n)()
+        rc = N.lib().th2d_cg_mdagm_batched(self.ctx.ptr, n, self._geoms, xs, bp, ks, k32,
+                                           C.byref(params), infos)
+        if rc not in (N.TH2D_OK, N.TH2D_ENOCONV, N.TH2D_EBREAKDOWN):
+            N.check(rc, "th2d_cg_mdagm_batched")
+        return [l.from_eo(d) for l, d in zip(self.lats, xs_d)], list(infos)
