@@ -36,10 +36,26 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--chains", type=int, default=8)
+    ap.add_argument("--chains", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dslash", action="store_true")
+    ap.add_argument("--only-headline", action="store_true",
+                    help="time the batched mixed-precision leg only (for profiling)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run a plain device copy of MIB MiB five times and exit (scale of "
+                         "the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 5}), flush=True)
 
 
 def run_hmc(H, N, size, precision, steps, warmup, chains=1, seed_links=1002):
@@ -129,6 +145,9 @@ def main():
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               "(independent chains shard trivially, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     from thirring2d_amd import _native as N
     from thirring2d_amd import hmc as H
 
@@ -136,6 +155,11 @@ def main():
     # headline: batched mixed precision
     dt_mx, res_mx, kms, kn, kbytes = run_hmc(H, N, size, N.PREC_MIXED, args.steps, args.warmup,
                                              chains=nch)
+    if args.only_headline:
+        print(json.dumps({"metric": "batched mixed HMC (profiling run)", "value": nch * args.steps / dt_mx,
+                          "unit": "traj/s", "chains": nch, "kernel_avg_ms": kms / max(kn, 1),
+                          "launches": kn}), flush=True)
+        return
     small = max(2, args.steps // 4)
     dt_b64, res_b64, *_ = run_hmc(H, N, size, N.PREC_F64, small, 1, chains=nch)
     dt_1mx, res_1mx, *_ = run_hmc(H, N, size, N.PREC_MIXED, small, 1, chains=1)

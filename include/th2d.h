@@ -150,11 +150,15 @@ int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *geoms, double
                           th2d_cg_info *infos);
 
 /* Tuning of the solver and of the operator kernel: tile shape (tt x tx
- * sites per workgroup tile, (tt+4)(tx+4) <= 1024; 0 0 = automatic), a cap
- * on the number of workgroups of a solver launch (0 = 8 per CU), and the
- * Dslash variant (0 direct, 1 LDS-tiled). */
+ * sites per workgroup tile, one of 16x64 8x64 16x32 8x32 16x16 8x16 8x8
+ * 4x8 4x4 2x4 2x2; 0 0 = the modelled default; a shape larger than a
+ * lattice of the batch falls back to the default), a cap on the number of
+ * workgroups of a solver launch (0 = one per tile), and the Dslash
+ * variant (0 direct, 1 LDS-tiled). */
 int th2d_set_cg_tile(th2d_ctx *ctx, int tt, int tx);
 int th2d_set_cg_max_blocks(th2d_ctx *ctx, int n);
+/* Tile shape used by the last solve on this context. */
+int th2d_cg_last_tile(th2d_ctx *ctx, int *tt, int *tx);
 /* Solver launches enqueued between two reads of the stop flags
  * (0 = the previous solve's iteration count, then 32). */
 int th2d_set_cg_chunk(th2d_ctx *ctx, int n);

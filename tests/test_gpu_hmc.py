@@ -111,18 +111,36 @@ def test_cg_reference_criterion(hmc, oracle):
     assert abs(info.iterations - it) <= 3
 
 
-@pytest.mark.parametrize("tile", [(4, 8), (8, 8), (8, 16), (16, 16), (16, 32)])
-def test_cg_tiles(hmc, oracle, tile):
-    """Every tile shape (several tiles per block included) solves the same system."""
+CG_SHAPES = [(16, 64), (8, 64), (16, 32), (8, 32), (16, 16), (8, 16), (8, 8), (4, 8), (4, 4),
+             (2, 4), (2, 2)]
+
+
+@pytest.mark.parametrize("prec", [0, 2])
+@pytest.mark.parametrize("tile", CG_SHAPES)
+def test_cg_tiles(hmc, oracle, tile, prec):
+    """Every compiled tile shape solves the same system, on a lattice that is
+    not a multiple of the larger tiles (partial tiles at the edges)."""
+    import ctypes as C
     from thirring2d_amd import _native as N
-    lat, L, A, b, _ = _setup(hmc, oracle, 40, 48, 51, m=0.2)
-    N.lib().th2d_set_cg_tile(lat.ctx.ptr, *tile)
+    lat, L, A, b, _ = _setup(hmc, oracle, 40, 72, 51, m=0.2)
+    assert N.lib().th2d_set_cg_tile(lat.ctx.ptr, *tile) == 0
     try:
-        x, info = lat.fmdm_invert_cg(b, hmc.cg_params(tol_rel=1e-12))
+        x, info = lat.fmdm_invert_cg(b, hmc.cg_params(tol_rel=1e-12 if prec == 0 else 1e-10,
+                                                       precision=prec))
+        tt, tx = C.c_int(), C.c_int()
+        N.lib().th2d_cg_last_tile(lat.ctx.ptr, C.byref(tt), C.byref(tx))
     finally:
         N.lib().th2d_set_cg_tile(lat.ctx.ptr, 0, 0)
+    assert (tt.value, tx.value) == tile
     r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
-    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-12
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= (1e-12 if prec == 0 else 1e-10)
+
+
+def test_cg_tile_rejected(hmc):
+    from thirring2d_amd import _native as N
+    ctx = hmc.Context.get()
+    assert N.lib().th2d_set_cg_tile(ctx.ptr, 12, 12) == N.TH2D_EARG
+    assert N.lib().th2d_set_cg_tile(ctx.ptr, 0, 0) == 0
 
 
 @pytest.mark.parametrize("n,prec", [(256, 0), (256, 2), (512, 2)])

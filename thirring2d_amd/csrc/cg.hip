@@ -31,7 +31,33 @@
 
 namespace {
 
-constexpr int MAXE = 4;  // halo-region elements per thread (E2 <= 1024)
+constexpr int CG_MAX_THREADS = 512;
+
+// Tile shapes of the step kernel (TT x TX sites, NT threads), compiled
+// one by one so that every index division is by a constant and every
+// element loop unrolls.
+struct ShapeEntry {
+  int tt, tx, nt;
+};
+constexpr ShapeEntry kShapes[] = {{16, 64, 512}, {8, 64, 256}, {16, 32, 256}, {8, 32, 256},
+                                  {16, 16, 256}, {8, 16, 256}, {8, 8, 128},   {4, 8, 64},
+                                  {4, 4, 64},    {2, 4, 64},   {2, 2, 64}};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+inline int shape_index(int tt, int tx)
+{
+  for (int k = 0; k < kNumShapes; k++)
+    if (kShapes[k].tt == tt && kShapes[k].tx == tx) return k;
+  return -1;
+}
+
+template <int TT_, int TX_, int NT_> struct Shape {
+  static constexpr int TT = TT_, TX = TX_, NT = NT_;
+  static constexpr int W2 = TX + 4, E2 = (TT + 4) * W2;  // 2-site halo region
+  static constexpr int W1 = TX + 2, E1 = (TT + 2) * W1;  // 1-site halo region
+  static constexpr int NE = (E2 + NT - 1) / NT, N1 = (E1 + NT - 1) / NT;
+  static constexpr int NO = (TT * TX + NT - 1) / NT;
+};
 
 struct CgChain {
   Geo G;
@@ -73,6 +99,14 @@ __device__ __forceinline__ int wrapn(int v, int n)
   while (v < 0) v += n;
   while (v >= n) v -= n;
   return v;
+}
+
+// Wrap of a coordinate at most one period outside [0, n) (tiles no larger
+// than the lattice).
+__device__ __forceinline__ int wrap1(int v, int n)
+{
+  v = v < 0 ? v + n : v;
+  return v >= n ? v - n : v;
 }
 
 template <typename T> struct Lds {
@@ -137,27 +171,54 @@ __device__ void apply_and_reduce(const Geo &G, const Lds<T> &c, int TT, int TX, 
   }
 }
 
+// Stores the tile's (r.r, r.w) as partial `tau` of `slot` (fixed order:
+// lanes, then waves).  c.red[0..15] must be free: its last reader is
+// thread 0 of the previous tile, before that tile's closing barrier.  The
+// closing barrier (LDS reuse by the next tile) is left out on the block's
+// last tile.
 template <typename T>
-__device__ __forceinline__ void write_tile_partial(const CgBatch<T> &a, const Lds<T> &c, int slot,
-                                                   int tau, double gacc, double dacc)
+__device__ __forceinline__ void write_tile_partial(const CgBatch<T> &a, double *red, int slot,
+                                                   int tau, double gacc, double dacc, bool last)
 {
-  double gs = block_sum(gacc, c.red);
-  __syncthreads();
-  double ds = block_sum(dacc, c.red + 16);
-  if (threadIdx.x == 0) {
-    a.part[((long)slot * a.ntiles + tau) * 2 + 0] = gs;
-    a.part[((long)slot * a.ntiles + tau) * 2 + 1] = ds;
+  gacc = wave_sum(gacc);
+  dacc = wave_sum(dacc);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[2 * wid] = gacc;
+    red[2 * wid + 1] = dacc;
   }
   __syncthreads();
+  if (threadIdx.x == 0) {
+    double g = 0.0, d = 0.0;
+    for (int k = 0; k < (int)((blockDim.x + 63) >> 6); k++) {
+      g += red[2 * k];
+      d += red[2 * k + 1];
+    }
+    a.part[((long)slot * a.ntiles + tau) * 2 + 0] = g;
+    a.part[((long)slot * a.ntiles + tau) * 2 + 1] = d;
+  }
+  if (!last) __syncthreads();
+}
+
+// Tile served by virtual block v.  Workgroups are dispatched round-robin
+// over the 8 XCDs (v mod 8 for a grid that is a multiple of 8); each XCD
+// gets a contiguous range of tiles, so the halo of a tile is mostly read
+// by blocks of the same XCD and shared through its L2.  A bijection of
+// [0, ntiles) for any ntiles.
+__device__ __forceinline__ int tile_of(int v, int ntiles)
+{
+  const int q = ntiles >> 3, rem = ntiles & 7, x = v & 7, k = v >> 3;
+  return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + k;
 }
 
 // r_0 = b, w_0 = A b, s_{-1} = p_{-1} = x_0 = 0; partials of slot 0.
 template <typename T>
-__global__ void __launch_bounds__(256) k_cg_init(CgBatch<T> a)
+__global__ void __launch_bounds__(CG_MAX_THREADS) k_cg_init(CgBatch<T> a)
 {
   using V2 = c2<T>;
   Lds<T> c = lds_view<T>(a.TT, a.TX);
-  for (int tau = blockIdx.x; tau < a.ntiles; tau += gridDim.x) {
+  for (int v = blockIdx.x; v < a.ntiles; v += gridDim.x) {
+    const int tau = tile_of(v, a.ntiles);
     const CgTile tl = a.tiles[tau];
     const CgChain ch = a.chains[tl.chain];
     const Geo &G = ch.G;
@@ -186,7 +247,7 @@ __global__ void __launch_bounds__(256) k_cg_init(CgBatch<T> a)
     __syncthreads();
     double gacc = 0.0, dacc = 0.0;
     apply_and_reduce(G, c, a.TT, a.TX, tl.t0, tl.x0, a.w0 + ch.voff, gacc, dacc);
-    write_tile_partial(a, c, 0, tau, gacc, dacc);
+    write_tile_partial(a, c.red, 0, tau, gacc, dacc, v + (int)gridDim.x >= a.ntiles);
     if (tau == ch.tbeg && threadIdx.x == 0) {
       a.state[tl.chain] = CgState{1.0, 1.0, 0, 0};
       a.res[tl.chain] = CgResult{0, TH2D_OK, 0.0, 0, 0};
@@ -195,21 +256,24 @@ __global__ void __launch_bounds__(256) k_cg_init(CgBatch<T> a)
 }
 
 // One iteration of every chain that has not stopped; reads the buffers of
-// `slot`, writes those of slot^1.
-template <typename T>
-__global__ void __launch_bounds__(256) k_cg_step(CgBatch<T> a, int slot)
+// `slot`, writes those of slot^1.  Every lattice of the batch is at least
+// S::TT x S::TX.
+template <typename T, typename S>
+__global__ void __launch_bounds__(S::NT) k_cg_step(CgBatch<T> a, int slot)
 {
   using V2 = c2<T>;
-  Lds<T> c = lds_view<T>(a.TT, a.TX);
+  constexpr int TT = S::TT, TX = S::TX, NT = S::NT, W2 = S::W2, W1 = S::W1;
+  constexpr int E2 = S::E2, E1 = S::E1, NE = S::NE;
+  constexpr int LDS_D = (3 * E2 + E1) * (int)sizeof(V2) / 8;
+  __shared__ double lds[LDS_D + 40];
+  V2 *R2 = (V2 *)lds, *Y1 = R2 + E2, *LKt = Y1 + E1, *LKx = LKt + E2;
+  double *red = lds + LDS_D;
+  const int tid = threadIdx.x;
   const int nxt = slot ^ 1;
-  for (int tau = blockIdx.x; tau < a.ntiles; tau += gridDim.x) {
+  for (int v = blockIdx.x; v < a.ntiles; v += gridDim.x) {
+    const bool last = v + (int)gridDim.x >= a.ntiles;
+    const int tau = tile_of(v, a.ntiles);
     const CgTile tl = a.tiles[tau];
-    // the stop flag of a chain is set during a launch by its first tile:
-    // read it once per block so that every thread takes the same branch
-    __syncthreads();
-    if (threadIdx.x == 0) c.red[40] = (double)a.res[tl.chain].done;
-    __syncthreads();
-    if (c.red[40] != 0.0) continue;
     const CgChain ch = a.chains[tl.chain];
     const Geo &G = ch.G;
     const int V = 2 * G.vh;
@@ -219,49 +283,58 @@ __global__ void __launch_bounds__(256) k_cg_step(CgBatch<T> a, int slot)
     V2 *rn_ = (slot ? a.r0 : a.r1) + ch.voff, *sn_ = (slot ? a.s0 : a.s1) + ch.voff;
     V2 *wn_ = (slot ? a.w0 : a.w1) + ch.voff;
     V2 *p = a.p + ch.voff, *x = a.x + ch.voff;
-    // 1. issue the loads: chain partials, halo-region vectors and links
-    double pg = 0.0, pd = 0.0;
-    for (int k = ch.tbeg + threadIdx.x; k < ch.tend; k += blockDim.x) {
-      pg += a.part[((long)slot * a.ntiles + k) * 2 + 0];
-      pd += a.part[((long)slot * a.ntiles + k) * 2 + 1];
-    }
-    V2 er[MAXE], ew[MAXE], es[MAXE], ep_[MAXE], ex_[MAXE];
-    int ei[MAXE];
-    bool eown[MAXE];
+    // 1. issue the loads: chain state, halo-region vectors and links, then
+    //    (wave 0) the chain's tile partials and its stop flag
+    const CgState st = a.state[(long)slot * a.nchains + tl.chain];
+    V2 er[NE], ew[NE], es[NE], ep_[NE], ex_[NE];
+    int ei[NE];
+    bool eown[NE];
 #pragma unroll
-    for (int q = 0; q < MAXE; q++) {
-      int e = threadIdx.x + q * blockDim.x;
+    for (int q = 0; q < NE; q++) {
+      const int e = tid + q * NT;
       ei[q] = -1;
       eown[q] = false;
-      if (e < c.E2) {
-        int et = e / c.W2, exx = e - et * c.W2;
-        int tg = tl.t0 - 2 + et, xg = tl.x0 - 2 + exx;
-        int i = eo_index(G, wrapn(tg, G.nt), wrapn(xg, G.nx));
+      if (E2 % NT == 0 || e < E2) {
+        const int et = e / W2, exx = e - et * W2;
+        const int tg = tl.t0 - 2 + et, xg = tl.x0 - 2 + exx;
+        const int i = eo_index(G, wrap1(tg, G.nt), wrap1(xg, G.nx));
         ei[q] = i;
         er[q] = rc[i];
         ew[q] = wc[i];
         es[q] = sc[i];
-        c.LKt[e] = K[i];
-        c.LKx[e] = K[V + i];
-        eown[q] = et >= 2 && et < a.TT + 2 && exx >= 2 && exx < a.TX + 2 && tg < G.nt &&
-                  xg < G.nx;
+        LKt[e] = K[i];
+        LKx[e] = K[V + i];
+        eown[q] = et >= 2 && et < TT + 2 && exx >= 2 && exx < TX + 2 && tg < G.nt && xg < G.nx;
         if (eown[q]) {
           ep_[q] = p[i];
           ex_[q] = x[i];
         }
       }
     }
-    // 2. scalars of iteration j, identical in every tile of the chain
-    double gs = block_sum(pg, c.red);
-    __syncthreads();
-    double ds = block_sum(pd, c.red + 16);
-    if (threadIdx.x == 0) {
-      c.red[32] = gs;
-      c.red[33] = ds;
+    // 2. scalars of iteration j, identical in every tile of the chain.  The
+    //    stop flag is set during a launch by the chain's first tile, so
+    //    lanes may see different values: lane 0's is broadcast.
+    if (tid < 64) {
+      const double *pp = a.part + (long)slot * a.ntiles * 2;
+      double pg = 0.0, pd = 0.0;
+      for (int k = ch.tbeg + tid; k < ch.tend; k += 64) {
+        pg += pp[2 * k];
+        pd += pp[2 * k + 1];
+      }
+      pg = wave_sum(pg);
+      pd = wave_sum(pd);
+      if (tid == 0) {
+        red[32] = pg;
+        red[33] = pd;
+        red[34] = (double)a.res[tl.chain].done;
+      }
     }
     __syncthreads();
-    const double gam = c.red[32], del = c.red[33];
-    const CgState st = a.state[(long)slot * a.nchains + tl.chain];
+    const double gam = red[32], del = red[33];
+    if (red[34] != 0.0) {
+      __syncthreads();  // red[32..34] is rewritten by the next tile
+      continue;
+    }
     const int j = st.j;
     bool stop = true;
     int status = TH2D_OK;
@@ -276,23 +349,23 @@ __global__ void __launch_bounds__(256) k_cg_step(CgBatch<T> a, int slot)
       else stop = false;
     }
     if (stop) {
-      if (tau == ch.tbeg && threadIdx.x == 0) a.res[tl.chain] = CgResult{j, status, gam, 1, 0};
+      if (tau == ch.tbeg && tid == 0) a.res[tl.chain] = CgResult{j, status, gam, 1, 0};
       __syncthreads();
       continue;
     }
-    if (tau == ch.tbeg && threadIdx.x == 0)
+    if (tau == ch.tbeg && tid == 0)
       a.state[(long)nxt * a.nchains + tl.chain] = CgState{gam, alpha, j + 1, 0};
     // 3. s_j, r_{j+1} on the halo region; p_j, x_{j+1} on owned sites
     const T ta = (T)alpha, tb = (T)beta;
 #pragma unroll
-    for (int q = 0; q < MAXE; q++) {
-      int e = threadIdx.x + q * blockDim.x;
-      if (ei[q] < 0) continue;
+    for (int q = 0; q < NE; q++) {
+      const int e = tid + q * NT;
+      if (!(E2 % NT == 0 || e < E2)) continue;
       V2 sn = cadd(ew[q], cscale(es[q], tb));
       V2 rn = csub(er[q], cscale(sn, ta));
-      c.R2[e] = rn;
+      R2[e] = rn;
       if (eown[q]) {
-        int i = ei[q];
+        const int i = ei[q];
         V2 pn = cadd(er[q], cscale(ep_[q], tb));
         p[i] = pn;
         x[i] = cadd(ex_[q], cscale(pn, ta));
@@ -301,37 +374,84 @@ __global__ void __launch_bounds__(256) k_cg_step(CgBatch<T> a, int slot)
       }
     }
     __syncthreads();
-    // 4./5. w_{j+1} = A r_{j+1}, partials of iteration j+1
-    double gacc = 0.0, dacc = 0.0;
-    apply_and_reduce(G, c, a.TT, a.TX, tl.t0, tl.x0, wn_, gacc, dacc);
-    write_tile_partial(a, c, nxt, tau, gacc, dacc);
+    // 4. y = M r_{j+1} on the 1-site halo region
+    const T m = (T)G.m, epm = (T)G.ep, emm = (T)G.em;
+#pragma unroll
+    for (int q = 0; q < S::N1; q++) {
+      const int e = tid + q * NT;
+      if (!(E1 % NT == 0 || e < E1)) continue;
+      const int ft = e / W1, fx = e - ft * W1;
+      const int c = (ft + 1) * W2 + (fx + 1);
+      V2 kt = LKt[c], kx = LKx[c], ktd = LKt[c - W2], kxd = LKx[c - 1];
+      V2 y = cscale(R2[c], m);
+      y = cadd(y, cscale(cmul(kt, R2[c + W2]), epm));
+      y = csub(y, cscale(cjmul(ktd, R2[c - W2]), emm));
+      y = cadd(y, cmul(kx, R2[c + 1]));
+      y = csub(y, cjmul(kxd, R2[c - 1]));
+      Y1[e] = y;
+    }
+    __syncthreads();
+    // 5. w_{j+1} = M^dag y on the owned sites, partials of iteration j+1
+    T gacc = 0, dacc = 0;
+#pragma unroll
+    for (int q = 0; q < S::NO; q++) {
+      const int e = tid + q * NT;
+      if (!((TT * TX) % NT == 0 || e < TT * TX)) continue;
+      const int lt = e / TX, lx = e - lt * TX;
+      const int tg = tl.t0 + lt, xg = tl.x0 + lx;
+      if (tg >= G.nt || xg >= G.nx) continue;
+      const int c1 = (lt + 1) * W1 + (lx + 1);
+      const int c2i = (lt + 2) * W2 + (lx + 2);
+      V2 kt = LKt[c2i], kx = LKx[c2i], ktd = LKt[c2i - W2], kxd = LKx[c2i - 1];
+      V2 w = cscale(Y1[c1], m);
+      w = csub(w, cscale(cmul(kt, Y1[c1 + W1]), emm));
+      w = cadd(w, cscale(cjmul(ktd, Y1[c1 - W1]), epm));
+      w = csub(w, cmul(kx, Y1[c1 + 1]));
+      w = cadd(w, cjmul(kxd, Y1[c1 - 1]));
+      wn_[eo_index(G, tg, xg)] = w;
+      V2 r = R2[c2i];
+      gacc += r.x * r.x + r.y * r.y;
+      dacc += r.x * w.x + r.y * w.y;
+    }
+    write_tile_partial(a, red, nxt, tau, (double)gacc, (double)dacc, last);
   }
 }
 
-struct TileChoice {
-  int tt, tx;
-};
-
-// Tile shape for a batch whose largest lattice is nt x nx holding
-// `sites` sites in all: near one tile per CU for a single lattice,
-// 16 x 16 when the batch fills the device anyway.
-TileChoice choose_tile(th2d_ctx *ctx, int nt, int nx, long sites)
+// Default tile shape for a batch: among the compiled shapes no larger
+// than any lattice of the batch, the one with the least modelled load
+// cost, loaded elements per owned site (2-site halo and padding of
+// lattices that are not multiples of the tile) times a coalescing factor
+// that favours wide rows (a row of the eo layout splits into two
+// half-rows), with a penalty when the batch has fewer tiles than CUs.
+// th2d_set_cg_tile overrides it (the autotuner measures instead).
+int choose_shape(th2d_ctx *ctx, int nch, const Geo *G)
 {
-  if (ctx->tile_t > 0 && ctx->tile_x > 0) return {ctx->tile_t, ctx->tile_x};
-  static const TileChoice cand[] = {{16, 16}, {8, 16}, {16, 32}, {8, 8}, {4, 8}};
-  if (sites >= 2L * 256 * ctx->num_cu) return {16, 16};
-  TileChoice best = {16, 16};
-  double best_score = 1e30;
-  for (auto c : cand) {
-    double n = double(sites) / (c.tt * c.tx);
-    double halo = double((c.tt + 4) * (c.tx + 4)) / (c.tt * c.tx);
-    double fill = n >= ctx->num_cu ? n / ctx->num_cu : double(ctx->num_cu) / n;
-    if (n > 2 * ctx->num_cu) fill *= 2.0;
-    double score = fill * halo * (c.tt * c.tx <= 256 ? 1.0 : 1.25);
-    if (nt * nx <= 1024 && sites <= 1024) score = (c.tt >= nt && c.tx >= nx) ? 0.0 : score;
-    if (score < best_score) {
-      best_score = score;
-      best = c;
+  int min_nt = 1 << 30, min_nx = 1 << 30;
+  for (int k = 0; k < nch; k++) {
+    min_nt = std::min(min_nt, G[k].nt);
+    min_nx = std::min(min_nx, G[k].nx);
+  }
+  if (ctx->tile_t > 0 && ctx->tile_x > 0) {
+    int s = shape_index(ctx->tile_t, ctx->tile_x);
+    if (s >= 0 && ctx->tile_t <= min_nt && ctx->tile_x <= min_nx) return s;
+  }
+  int best = -1;
+  double best_cost = 1e300;
+  for (int s = 0; s < kNumShapes; s++) {
+    const ShapeEntry c = kShapes[s];
+    if (c.tt > min_nt || c.tx > min_nx) continue;
+    double tiles = 0.0, loaded = 0.0, owned = 0.0;
+    for (int k = 0; k < nch; k++) {
+      double n = double((G[k].nt + c.tt - 1) / c.tt) * double((G[k].nx + c.tx - 1) / c.tx);
+      tiles += n;
+      loaded += n * (c.tt + 4) * (c.tx + 4);
+      owned += 2.0 * G[k].vh;
+    }
+    double cost = loaded / owned * (1.0 + 48.0 / (c.tx + 4));
+    if (tiles < ctx->num_cu) cost *= ctx->num_cu / tiles;
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = s;
     }
   }
   return best;
@@ -342,6 +462,32 @@ template <typename T> size_t lds_bytes(int tt, int tx)
   size_t e2 = (size_t)(tt + 4) * (tx + 4), e1 = (size_t)(tt + 2) * (tx + 2);
   return sizeof(c2<T>) * (3 * e2 + e1) + sizeof(double) * 48;
 }
+
+template <typename T, int K>
+void launch_shape(int nblocks, hipStream_t st, const CgBatch<T> &a, int slot)
+{
+  using S = Shape<kShapes[K].tt, kShapes[K].tx, kShapes[K].nt>;
+  hipLaunchKernelGGL((k_cg_step<T, S>), dim3(nblocks), dim3(S::NT), 0, st, a, slot);
+}
+
+template <typename T>
+void launch_step(int shape, int nblocks, hipStream_t st, const CgBatch<T> &a, int slot)
+{
+  switch (shape) {
+  case 0: launch_shape<T, 0>(nblocks, st, a, slot); break;
+  case 1: launch_shape<T, 1>(nblocks, st, a, slot); break;
+  case 2: launch_shape<T, 2>(nblocks, st, a, slot); break;
+  case 3: launch_shape<T, 3>(nblocks, st, a, slot); break;
+  case 4: launch_shape<T, 4>(nblocks, st, a, slot); break;
+  case 5: launch_shape<T, 5>(nblocks, st, a, slot); break;
+  case 6: launch_shape<T, 6>(nblocks, st, a, slot); break;
+  case 7: launch_shape<T, 7>(nblocks, st, a, slot); break;
+  case 8: launch_shape<T, 8>(nblocks, st, a, slot); break;
+  case 9: launch_shape<T, 9>(nblocks, st, a, slot); break;
+  default: launch_shape<T, 10>(nblocks, st, a, slot); break;
+  }
+}
+static_assert(kNumShapes == 11, "launch_step lists every shape");
 
 struct InnerResult {
   int iters;
@@ -356,15 +502,12 @@ int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T>
                 const c2<T> *const *K, const double *stop_rr, const int *max_iter, c2<T> **xout,
                 InnerResult *res)
 {
-  int max_nt = 0, max_nx = 0;
-  long sites = 0;
-  for (int c = 0; c < nch; c++) {
-    max_nt = std::max(max_nt, G[c].nt);
-    max_nx = std::max(max_nx, G[c].nx);
-    sites += 2L * G[c].vh;
-  }
-  TileChoice tc = choose_tile(ctx, max_nt, max_nx, sites);
-  if ((tc.tt + 4) * (tc.tx + 4) > MAXE * 256) return TH2D_EARG;
+  const int shape = choose_shape(ctx, nch, G);
+  if (shape < 0) return TH2D_EARG;
+  const ShapeEntry tc = kShapes[shape];
+  const int nthreads = tc.nt;
+  ctx->cg_last_tile[0] = tc.tt;
+  ctx->cg_last_tile[1] = tc.tx;
   // host tables
   std::vector<CgChain> ch(nch);
   std::vector<CgTile> tiles;
@@ -426,10 +569,13 @@ int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T>
   memcpy((char *)ctx->redh.ptr + chb, tiles.data(), sizeof(CgTile) * ntiles);
   TH2D_HIP(hipMemcpyAsync(dch, ctx->redh.ptr, chb + tlb, hipMemcpyHostToDevice, ctx->stream));
   const size_t lds = lds_bytes<T>(tc.tt, tc.tx);
-  const int cap = ctx->cg_max_blocks > 0 ? ctx->cg_max_blocks : 8 * ctx->num_cu;
-  const int nblocks = std::max(1, std::min(ntiles, cap));
+  // one block per tile unless capped; a capped grid is a multiple of 8 so
+  // that a block keeps its XCD over the grid-stride loop
+  int nblocks = ntiles;
+  if (ctx->cg_max_blocks > 0 && ctx->cg_max_blocks < ntiles)
+    nblocks = std::max(8, ctx->cg_max_blocks & ~7);
   if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k0, ctx->stream));
-  hipLaunchKernelGGL(k_cg_init<T>, dim3(nblocks), dim3(256), lds, ctx->stream, a);
+  hipLaunchKernelGGL(k_cg_init<T>, dim3(nblocks), dim3(nthreads), lds, ctx->stream, a);
   TH2D_HIP(hipGetLastError());
   std::vector<CgResult> h(nch);
   long launched = 0;
@@ -439,8 +585,7 @@ int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T>
   for (int c = 0; c < nch; c++) most = std::max(most, max_iter[c]);
   for (;;) {
     for (int k = 0; k < chunk; k++)
-      hipLaunchKernelGGL(k_cg_step<T>, dim3(nblocks), dim3(256), lds, ctx->stream, a,
-                         (int)((launched + k) & 1));
+      launch_step(shape, nblocks, ctx->stream, a, (int)((launched + k) & 1));
     TH2D_HIP(hipGetLastError());
     launched += chunk;
     TH2D_HIP(hipMemcpyAsync(ctx->redh.ptr, a.res, sizeof(CgResult) * nch, hipMemcpyDeviceToHost,
@@ -513,10 +658,18 @@ __global__ void k_axpy_f64(double2 *y, const double2 *x, long n, double a)
 
 extern "C" int th2d_set_cg_tile(th2d_ctx *ctx, int tt, int tx)
 {
-  if (!ctx || tt < 0 || tx < 0 || (tx & 1)) return TH2D_EARG;
-  if ((tt || tx) && (tt + 4) * (tx + 4) > MAXE * 256) return TH2D_EARG;
+  if (!ctx || tt < 0 || tx < 0) return TH2D_EARG;
+  if ((tt || tx) && shape_index(tt, tx) < 0) return TH2D_EARG;
   ctx->tile_t = tt;
   ctx->tile_x = tx;
+  return TH2D_OK;
+}
+
+extern "C" int th2d_cg_last_tile(th2d_ctx *ctx, int *tt, int *tx)
+{
+  if (!ctx || !tt || !tx) return TH2D_EARG;
+  *tt = ctx->cg_last_tile[0];
+  *tx = ctx->cg_last_tile[1];
   return TH2D_OK;
 }
 

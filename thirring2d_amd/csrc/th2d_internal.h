@@ -50,10 +50,11 @@ struct th2d_ctx {
   th2d_buf cg64, cg32, cgmix, red, redh, hmc, force, bag, misc;
   // tile configuration of the persistent solver (0 = autotuned default)
   int tile_t = 0, tile_x = 0;
-  int cg_max_blocks = 0;  // 0 = automatic (4 per CU at most)
+  int cg_max_blocks = 0;  // 0 = one workgroup per tile
   int cg_chunk = 0;       // step launches per stop-flag check (0 = 32, doubling)
   long kiters = 0;
   int cg_last_iters[2] = {0, 0};  // FP32, FP64 inner solves
+  int cg_last_tile[2] = {0, 0};   // tile shape of the last inner solve
 };
 
 int th2d_buf_reserve(th2d_buf &b, size_t bytes);
@@ -141,6 +142,30 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
   if (threadIdx.x == 0) {
     int nw = (blockDim.x + 63) >> 6;
     for (int k = 0; k < nw; k++) s += sh[k];
+  }
+  return s;
+}
+
+// Block sums of two values at once (fixed order as block_sum); results
+// valid in thread 0.  `sh` holds 2 * blockDim/64 doubles.
+__device__ __forceinline__ double2 block_sum2(double a, double b, double *sh)
+{
+  a = wave_sum(a);
+  b = wave_sum(b);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    sh[2 * wid] = a;
+    sh[2 * wid + 1] = b;
+  }
+  __syncthreads();
+  double2 s = make_double2(0.0, 0.0);
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + 63) >> 6;
+    for (int k = 0; k < nw; k++) {
+      s.x += sh[2 * k];
+      s.y += sh[2 * k + 1];
+    }
   }
   return s;
 }
