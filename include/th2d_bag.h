@@ -140,6 +140,14 @@ int th2d_bag_calc_dinv(th2d_ctx *ctx, const th2d_bag_geom *g, double *Dinv);
 int th2d_bag_small_weights(th2d_ctx *ctx, const th2d_bag_geom *g, const double *Dinv,
                            const unsigned char *cfg, long count, double *w);
 
+/* Exact enumeration of all link/monomer configurations (tests/
+ * thirring_brute.c:402-494): Z[nl*(V+1) + nm] += w and Zabs[...] += |w|
+ * over every configuration with nl links and nm monomers, w = det_e det_o
+ * evaluated on the device (host arrays of (V/2+1)(V+1) doubles; V/2 <= 16).
+ * *visited receives the number of configurations. */
+int th2d_bag_brute_enumerate(th2d_ctx *ctx, const th2d_bag_geom *g, double *Z, double *Zabs,
+                             long *visited);
+
 #ifdef __cplusplus
 }
 #endif

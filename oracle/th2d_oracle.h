@@ -160,8 +160,9 @@ int orc_determinant_mu(const orc_bag_lattice *B, const int *field, double mu,
                        double *logabs, int *sign);
 
 /* Free propagator blocks in the reference layout (fermion_matrix.c:191-351,
- * tests/thirring_brute.c:190-264): Dinv[e*V/2 + o] = (D^-1)_{o,e} and
- * Dinv[V^2/4 + o*V/2 + e] = (D^-1)_{e,o}, with D the unmasked matrix of
+ * tests/thirring_brute.c:190-264): Dinv[e*V/2 + o] = (D^-1)_{e,o} and
+ * Dinv[V^2/4 + o*V/2 + e] = (D^-1)_{o,e} (fermion_matrix.c:237-238,
+ * 290-291), with D the unmasked matrix of
  * fM_index.  Computed by dense LU inversion of the two eo blocks. */
 int orc_calc_Dinv(const orc_bag_lattice *B, double *Dinv);
 

@@ -142,6 +142,7 @@ SIGNATURES = {
     "th2d_bag_views": (I, [P, C.POINTER(P), C.POINTER(P), C.POINTER(I)]),
     "th2d_bag_calc_dinv": (I, [P, C.POINTER(BagGeom), P]),
     "th2d_bag_small_weights": (I, [P, C.POINTER(BagGeom), P, P, L, P]),
+    "th2d_bag_brute_enumerate": (I, [P, C.POINTER(BagGeom), P, P, C.POINTER(L)]),
 }
 
 _lib = None

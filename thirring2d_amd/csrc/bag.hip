@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <vector>
 
 #include "../../include/th2d_bag.h"
@@ -887,4 +888,115 @@ extern "C" int th2d_bag_small_weights(th2d_ctx *c, const th2d_bag_geom *g, const
                      c->stream, g->nt, g->nx, Dinv, cfg, count, w);
   TH2D_HIP(hipGetLastError());
   return TH2D_OK;
+}
+
+// Exact enumeration of every link/monomer configuration (tests/
+// thirring_brute.c:402-494): depth-first on the host in the reference's
+// order (link in direction t, then x, then a monomer at each site >= the
+// last placed one), weights det_e det_o on the device in chunks, sums per
+// (links, monomers) in enumeration order.
+namespace {
+struct BruteEnum {
+  BagGeo G;
+  int V;
+  std::vector<int> f;
+  std::vector<unsigned char> chunk;
+  std::vector<int> bins;
+  long n = 0, cap = 0, visited = 0;
+  int rc = TH2D_OK;
+  std::function<int()> flush;
+  void visit()
+  {
+    int nl = 0, nm = 0;
+    for (int s = 0; s < V; s++) {
+      nl += f[s] == 2 || f[s] == 3;
+      nm += f[s] == 1;
+    }
+    unsigned char *dst = chunk.data() + (size_t)n * V;
+    for (int s = 0; s < V; s++) dst[s] = (unsigned char)f[s];
+    bins[n] = nl * (V + 1) + nm;
+    n++;
+    visited++;
+    if (n == cap && rc == TH2D_OK) rc = flush();
+  }
+  void rec(int i0)
+  {
+    visit();
+    for (int i = i0; i < V && rc == TH2D_OK; i++) {
+      if (f[i]) continue;
+      int t = i / G.nx, x = i - t * G.nx;
+      for (int nu = 0; nu < 2; nu++) {
+        int t2, x2;
+        bag_nb(G, t, x, nu, t2, x2);
+        if (x2 < 0) continue;
+        int s2 = t2 * G.nx + x2;
+        if (f[s2]) continue;
+        f[i] = 2 + nu;
+        f[s2] = 4 + nu;
+        rec(i);
+        f[i] = 0;
+        f[s2] = 0;
+      }
+      f[i] = 1;
+      rec(i);
+      f[i] = 0;
+    }
+  }
+};
+}  // namespace
+
+extern "C" int th2d_bag_brute_enumerate(th2d_ctx *c, const th2d_bag_geom *g, double *Z,
+                                        double *Zabs, long *visited)
+{
+  TH2D_TRY(check_bag_geom(g));
+  if (!c || !Z || !Zabs) return TH2D_EARG;
+  const int V = g->nt * g->nx, h = V / 2;
+  if (h > SMALL_N) return TH2D_EARG;
+  const long cap = 1L << 20;
+  double *Dinv = nullptr, *w = nullptr;
+  unsigned char *cfg = nullptr;
+  if (hipMalloc(&Dinv, sizeof(double) * 2 * h * h) != hipSuccess ||
+      hipMalloc(&w, sizeof(double) * cap) != hipSuccess ||
+      hipMalloc(&cfg, (size_t)cap * V) != hipSuccess) {
+    if (Dinv) (void)hipFree(Dinv);
+    if (w) (void)hipFree(w);
+    return TH2D_ENOMEM;
+  }
+  int rc = th2d_bag_calc_dinv(c, g, Dinv);
+  const size_t nbins = (size_t)(h + 1) * (V + 1);
+  for (size_t k = 0; k < nbins; k++) Z[k] = Zabs[k] = 0.0;
+  if (rc == TH2D_OK) {
+    BruteEnum E;
+    E.G = bag_geo(g);
+    E.V = V;
+    E.f.assign(V, 0);
+    E.cap = cap;
+    E.chunk.resize((size_t)cap * V);
+    E.bins.resize(cap);
+    std::vector<double> hw(cap);
+    E.flush = [&]() -> int {
+      if (!E.n) return TH2D_OK;
+      TH2D_HIP(hipMemcpyAsync(cfg, E.chunk.data(), (size_t)E.n * V, hipMemcpyHostToDevice,
+                              c->stream));
+      TH2D_TRY(th2d_bag_small_weights(c, g, Dinv, cfg, E.n, w));
+      TH2D_HIP(hipMemcpyAsync(hw.data(), w, sizeof(double) * E.n, hipMemcpyDeviceToHost,
+                              c->stream));
+      TH2D_HIP(hipStreamSynchronize(c->stream));
+      for (long k = 0; k < E.n; k++) {
+        Z[E.bins[k]] += hw[k];
+        Zabs[E.bins[k]] += fabs(hw[k]);
+      }
+      E.n = 0;
+      return TH2D_OK;
+    };
+    E.rec(0);
+    if (E.rc == TH2D_OK) E.rc = E.flush();
+    rc = E.rc;
+    if (visited) *visited = E.visited;
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(Dinv);
+  (void)hipFree(w);
+  (void)hipFree(cfg);
+  return rc;
 }

@@ -33,12 +33,17 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Mfma;
+// Operand layout of both 16x16x4 instructions (lane l): A[l%16][l/16],
+// B[l/16][l%16].  Result register r of lane l holds row row(l, r) and
+// column l%16 -- rows are blocked by lane group in FP32 (4*(l/16) + r)
+// but interleaved in FP64 (4*r + l/16), as measured on gfx950.
 template <> struct Mfma<double> {
   using acc = v4d;
   static __device__ __forceinline__ acc op(double a, double b, acc c)
   {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + (lane >> 4); }
 };
 template <> struct Mfma<float> {
   using acc = v4f;
@@ -46,13 +51,13 @@ template <> struct Mfma<float> {
   {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 // C[i][j] += alpha * sum_k A[i][k] B[k][j] for one BM x BN tile at
 // (i0, j0); rows >= M, cols >= N, k >= K are skipped.  256 threads: wave w
-// owns the 32x32 quadrant (w/2, w%2) as 2x2 MFMA blocks of 16x16.
-// MFMA 16x16x4 operand layout (lane l): A[l%16][l/16], B[l/16][l%16],
-// result register r: D[4*(l/16)+r][l%16].
+// owns the 32x32 quadrant (w/2, w%2) as 2x2 MFMA blocks of 16x16; a
+// thread's results sit at rows Mfma<T>::row(lane, r) of each block.
 template <typename T, bool USE_MFMA>
 __device__ void gemm_tile(const T *A, long lda, const T *B, long ldb, T *C, long ldc, int M, int N,
                           int K, T alpha, int i0, int j0, T *sA, T *sB)
@@ -92,12 +97,12 @@ __device__ void gemm_tile(const T *A, long lda, const T *B, long ldb, T *C, long
           for (int b = 0; b < 2; b++) acc[a][b] = Mfma<T>::op(av[a], bv[b], acc[a][b]);
       }
     } else {
-      // thread owns the 4x4 block matching the MFMA result layout:
-      // rows wr + a*16 + 4*(lane/16) + r, cols wc + b*16 + lane%16
+      // thread owns the entries of the MFMA result layout:
+      // rows wr + a*16 + row(lane, r), cols wc + b*16 + lane%16
       for (int kk = 0; kk < BK; kk++)
         for (int a = 0; a < 2; a++)
           for (int r = 0; r < 4; r++) {
-            T av = sA[(wr + a * 16 + 4 * (lane >> 4) + r) * (BK + 1) + kk];
+            T av = sA[(wr + a * 16 + Mfma<T>::row(lane, r)) * (BK + 1) + kk];
             for (int b = 0; b < 2; b++)
               vacc[a * 2 + b][r] += av * sB[kk * (BN + 1) + wc + b * 16 + (lane & 15)];
           }
@@ -106,7 +111,7 @@ __device__ void gemm_tile(const T *A, long lda, const T *B, long ldb, T *C, long
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++)
       for (int r = 0; r < 4; r++) {
-        int gi = i0 + wr + a * 16 + 4 * (lane >> 4) + r;
+        int gi = i0 + wr + a * 16 + Mfma<T>::row(lane, r);
         int gj = j0 + wc + b * 16 + (lane & 15);
         if (gi < M && gj < N) {
           T v = USE_MFMA ? acc[a][b][r] : vacc[a * 2 + b][r];
