@@ -226,6 +226,16 @@ int th2d_hmc_trajectory_batched(th2d_ctx *ctx, int nch, const th2d_geom *geos,
                                 unsigned long long seed, const unsigned long long *counters,
                                 th2d_hmc_result *res);
 
+/* ---- dense complex matrix ---------------------------------------------- */
+/* The V x V complex fermion matrix, column-major M[row + col*V] in natural
+ * site order (fermion_matrix, hmc.c:269-310), from folded links K (device
+ * output of 2 V^2 doubles). */
+int th2d_fermion_matrix(th2d_ctx *ctx, const th2d_geom *g, const double *K, double *M);
+/* det of a column-major complex n x n device matrix (destroyed) by
+ * partial-pivot LU (determinant, hmc.c:314-336): log|det| and arg(det);
+ * TH2D_ESINGULAR (log|det| = -inf) at an exactly zero pivot. */
+int th2d_zdet(th2d_ctx *ctx, int n, double *M, double *logabs, double *phase);
+
 /* Gaussian N(0,1) link angles from the device generator (configs: A~N(0,1)). */
 int th2d_random_links(th2d_ctx *ctx, const th2d_geom *g, double *A, unsigned long long seed);
 

@@ -140,6 +140,17 @@ int th2d_bag_calc_dinv(th2d_ctx *ctx, const th2d_bag_geom *g, double *Dinv);
 int th2d_bag_small_weights(th2d_ctx *ctx, const th2d_bag_geom *g, const double *Dinv,
                            const unsigned char *cfg, long count, double *w);
 
+/* Matrix element fM_index(t1,x1,t2,x2,mu) (fermion_matrix.c:28-64,
+ * 152-182) of the lattice g at the given mu (host, no device work). */
+int th2d_bag_matrix_element(const th2d_bag_geom *g, int t1, int x1, int t2, int x2, double mu,
+                            double *out);
+
+/* log|det| and sign of the matrix restricted to the free sites of a
+ * configuration (determinant_mu, full_determinant.c:144-186; field host
+ * int32[V]) by LU on the device; sign 0 and -inf at an exact zero pivot. */
+int th2d_bag_determinant(th2d_ctx *ctx, const th2d_bag_geom *g, const int *field_host,
+                         double *logabs, int *sign);
+
 /* Exact enumeration of all link/monomer configurations (tests/
  * thirring_brute.c:402-494): Z[nl*(V+1) + nm] += w and Zabs[...] += |w|
  * over every configuration with nl links and nm monomers, w = det_e det_o

@@ -1,0 +1,354 @@
+"""The reference's own function signatures (include/thirring_hmc.h,
+include/thirring_bag.h) called the way the reference's programs call them,
+with jagged arrays and globals, against the CPU oracle.
+
+Tolerances as the native tests: operator <= 1e-12 relative, solver
+solutions at the requested tolerance, determinants 1e-10 relative in
+log|det| and 1e-10 in phase, bag ratios 1e-9 relative; the random streams
+and accept decisions are identical.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from jagged import Links, Rows, load
+
+pytestmark = pytest.mark.gpu
+
+P, D, I = C.c_void_p, C.c_double, C.c_int
+
+
+class Cplx(C.Structure):
+    """double complex returned by value (two SSE eightbytes, as a struct of
+    two doubles in the x86-64 calling convention)."""
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def hmc_lib():
+    lib = load("libthirring_hmc.so")
+    for n in ("fm_mul", "fm_conjugate_mul", "fmdm_mul", "fmdm_invert_cg", "fm_invert_cg",
+              "stochastic_vector", "fermion_matrix", "update_gauge", "update_puregauge_hb"):
+        getattr(lib, n).restype = None
+    for n in ("calc_gauge_action", "gauge_step", "random_pseudofermion", "pseudofermion_action",
+              "stochastic_md_action", "random_momentum", "momentum_step"):
+        getattr(lib, n).restype = D
+    lib.gauge_step.argtypes = [P, P, D]
+    lib.momentum_step.argtypes = [P, P, P, P, D]
+    lib.determinant.restype = Cplx
+    lib.thirring_hmc_set_cg.argtypes = [D, D, I, I]
+    lib.thirring_hmc_set_md.argtypes = [I, D]
+    lib.thirring_hmc_last_trajectory.argtypes = [C.POINTER(I), C.POINTER(D), C.POINTER(I)]
+    lib.seed_mersenne.argtypes = [C.c_long]
+    return lib
+
+
+def _globals(lib, m, g, mu):
+    C.c_double.in_dll(lib, "m").value = m
+    C.c_double.in_dll(lib, "g").value = g
+    C.c_double.in_dll(lib, "mu").value = mu
+
+
+def _setup(lib, oracle, nt, nx, seed, m=0.1, g=0.5, mu=0.0):
+    rng = np.random.default_rng(seed)
+    V = nt * nx
+    A = rng.normal(size=(V, 2))
+    _globals(lib, m, g, mu)
+    assert lib.thirring_hmc_init(nt, nx) == 0
+    L = oracle.lattice(nt, nx, m=m, g=g, mu=mu)
+    v = rng.normal(size=V) + 1j * rng.normal(size=V)
+    return L, A, v
+
+
+@pytest.mark.parametrize("nt,nx,mu", [(8, 8, 0.0), (12, 16, 0.3)])
+def test_fm_mul_signatures(hmc_lib, oracle, nt, nx, mu):
+    L, A, v = _setup(hmc_lib, oracle, nt, nx, 1 + nt, mu=mu)
+    F = Links(nt, nx, A)
+    vin, vout = Rows(nt, nx, np.complex128, v), Rows(nt, nx, np.complex128)
+    hmc_lib.fm_mul(vin.ptr, vout.ptr, F.ptr)
+    assert _rel(vout.natural(), oracle.fm_mul(L, A, v)) <= 1e-12
+    hmc_lib.fm_conjugate_mul(vin.ptr, vout.ptr, F.ptr)
+    assert _rel(vout.natural(), oracle.fm_dag_mul(L, A, v)) <= 1e-12
+    hmc_lib.fmdm_mul(vin.ptr, vout.ptr, F.ptr)
+    want = oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, v))
+    assert _rel(vout.natural(), want) <= 1e-12
+    assert hmc_lib.thirring_hmc_last_status() == 0
+
+
+def test_invert_signatures(hmc_lib, oracle):
+    nt, nx = 16, 16
+    L, A, v = _setup(hmc_lib, oracle, nt, nx, 3)
+    F = Links(nt, nx, A)
+    vin, vout = Rows(nt, nx, np.complex128, v), Rows(nt, nx, np.complex128)
+    hmc_lib.fmdm_invert_cg(vin.ptr, vout.ptr, F.ptr)  # reference stop: r.r < 1e-30
+    x = vout.natural()
+    xo, it, st = oracle.cg_mdm(L, A, v, tol_abs_rr=1e-30, tol_rel=0.0)
+    assert _rel(x, xo) <= 1e-12
+    hmc_lib.fm_invert_cg(vin.ptr, vout.ptr, F.ptr)  # M^-1 v
+    assert _rel(oracle.fm_mul(L, A, vout.natural()), v) <= 1e-10
+    assert hmc_lib.thirring_hmc_last_status() == 0
+
+
+def test_fermion_matrix_and_determinant(hmc_lib, oracle):
+    nt, nx = 6, 8
+    L, A, v = _setup(hmc_lib, oracle, nt, nx, 5, mu=0.2)
+    F = Links(nt, nx, A)
+    A_glob = C.c_void_p.in_dll(hmc_lib, "A")
+    A_glob.value = F.ptr.value
+    try:
+        V = nt * nx
+        M = np.zeros(V * V, np.complex128)
+        hmc_lib.fermion_matrix(M.ctypes.data_as(P))
+        Mo = oracle.fermion_matrix(L, A)
+        assert _rel(M.reshape(V, V, order="F"), Mo) <= 1e-14
+        det = hmc_lib.determinant()
+        z = complex(det.re, det.im)
+        la, ph, info = oracle.zlogdet(Mo)
+        assert info == 0
+        assert abs(np.log(abs(z)) - la) <= 1e-10 * max(1.0, abs(la))
+        assert abs(np.angle(z * np.exp(-1j * ph))) <= 1e-10
+        s, l = np.linalg.slogdet(Mo)
+        assert abs(l - la) <= 1e-10 * max(1.0, abs(l))
+    finally:
+        A_glob.value = None
+
+
+def test_random_fields_follow_the_reference_stream(hmc_lib, oracle):
+    """random_pseudofermion, random_momentum, stochastic_vector draw the
+    reference's MT stream in its order and normalisation (D4/D5)."""
+    nt, nx = 8, 8
+    L, A, _ = _setup(hmc_lib, oracle, nt, nx, 7)
+    F = Links(nt, nx, A)
+    hmc_lib.seed_mersenne(2001)
+    rng = oracle.Rng(2001)
+    v = Rows(nt, nx, np.complex128)
+    S = hmc_lib.random_pseudofermion(v.ptr, F.ptr)
+    So, xi, psi = oracle.random_pseudofermion(L, rng, A)
+    assert abs(S - So) <= 1e-14 * So
+    assert _rel(v.natural(), psi) <= 1e-12
+    mom = Links(nt, nx)
+    Sm = hmc_lib.random_momentum(mom.ptr)
+    Smo, momo = oracle.random_momentum(L, rng)
+    assert abs(Sm - Smo) <= 1e-14 * Smo and np.array_equal(mom.natural(), momo)
+    st = Rows(nt, nx, np.complex128)
+    hmc_lib.stochastic_vector(st.ptr)
+    assert np.array_equal(st.natural(), oracle.stochastic_vector(L, rng))
+
+
+def test_actions_and_steps(hmc_lib, oracle):
+    nt, nx = 8, 12
+    L, A, v = _setup(hmc_lib, oracle, nt, nx, 9, mu=0.25)
+    hmc_lib.thirring_hmc_set_cg(1e-12, 0.0, 100000, 0)
+    try:
+        F = Links(nt, nx, A)
+        psi = Rows(nt, nx, np.complex128, v)
+        rng = np.random.default_rng(10)
+        stv = rng.normal(size=nt * nx) + 1j * rng.normal(size=nt * nx)
+        st = Rows(nt, nx, np.complex128, stv)
+        assert abs(hmc_lib.calc_gauge_action(F.ptr) - oracle.gauge_action(L, A)) <= 1e-12 * abs(
+            oracle.gauge_action(L, A))
+        Spf = hmc_lib.pseudofermion_action(psi.ptr, F.ptr)
+        Spfo, _ = oracle.pseudofermion_action(L, A, v, tol_rel=1e-12)
+        assert abs(Spf - Spfo) <= 1e-10 * abs(Spfo)
+        Smd = hmc_lib.stochastic_md_action(st.ptr, F.ptr)
+        assert abs(Smd - oracle.stochastic_md_action(L, A, stv)) <= 1e-12 * abs(Smd)
+        mom0 = rng.normal(size=(nt * nx, 2))
+        mom = Links(nt, nx, mom0)
+        hmc_lib.momentum_step(mom.ptr, F.ptr, psi.ptr, st.ptr, 0.05)
+        momo, _, _ = oracle.momentum_step(L, mom0, A, v, stv, 0.05, tol_rel=1e-12)
+        assert _rel(mom.natural(), momo) <= 1e-9
+        hmc_lib.gauge_step(F.ptr, mom.ptr, 0.05)
+        assert _rel(F.natural(), oracle.gauge_step(L, A, mom.natural(), 0.05)) <= 1e-14
+        assert hmc_lib.thirring_hmc_last_status() == 0
+    finally:
+        hmc_lib.thirring_hmc_set_cg(0.0, 1e-30, 100000, 0)
+
+
+def test_update_gauge_matches_oracle(hmc_lib, oracle):
+    """update_gauge draws the reference's stream (pseudofermion, momentum,
+    stochastic vector, accept) and runs the trajectory on the GPU: same
+    dH, accept decisions and fields as the oracle, trajectory by trajectory."""
+    nt, nx = 8, 8
+    L, A, _ = _setup(hmc_lib, oracle, nt, nx, 11, m=0.3, g=0.5)
+    hmc_lib.thirring_hmc_set_cg(1e-12, 0.0, 100000, 0)
+    hmc_lib.thirring_hmc_set_md(20, 1.0)
+    try:
+        F = Links(nt, nx, A)
+        hmc_lib.seed_mersenne(4001)
+        rng = oracle.Rng(4001)
+        Ao = A.copy()
+        n_acc = 0
+        for _ in range(4):
+            hmc_lib.update_gauge(F.ptr)
+            acc, dH, its = I(), D(), I()
+            hmc_lib.thirring_hmc_last_trajectory(C.byref(acc), C.byref(dH), C.byref(its))
+            acco, dSo, _, _, Ao = oracle.update_gauge(L, rng, Ao, nsteps=20, tol_rel=1e-12)
+            assert acc.value == acco
+            assert abs(dH.value - dSo) <= 1e-7 * max(1.0, abs(dSo))
+            assert _rel(F.natural(), Ao) <= 1e-9
+            n_acc += acco
+        assert hmc_lib.thirring_hmc_last_status() == 0
+        assert n_acc >= 1
+    finally:
+        hmc_lib.thirring_hmc_set_cg(0.0, 1e-30, 100000, 0)
+        hmc_lib.thirring_hmc_set_md(10, 1.0)
+
+
+def test_puregauge_matches_oracle(hmc_lib, oracle):
+    nt, nx = 8, 10
+    L, A, _ = _setup(hmc_lib, oracle, nt, nx, 13)
+    F = Links(nt, nx, A)
+    hmc_lib.seed_mersenne(5)
+    rng = oracle.Rng(5)
+    Ao = A
+    for _ in range(3):
+        hmc_lib.update_puregauge_hb(F.ptr)
+        Ao = oracle.puregauge_sweep(L, rng, Ao)
+    assert _rel(F.natural(), Ao) <= 1e-15
+
+
+# ------------------------------------------------------------- bag family --
+@pytest.fixture(scope="module")
+def bag_lib():
+    lib = load("libthirring_bag.so")
+    for n in ("fM_index", "determinant", "determinant_mu", "det_added_link", "det_removed_link",
+              "det_moved_monomer"):
+        getattr(lib, n).restype = D
+    lib.fM_index.argtypes = [I, I, I, I, D]
+    lib.determinant_mu.argtypes = [D]
+    for n in ("fM", "fM_transpose", "cg_propagator", "calc_Dinv", "update_background",
+              "new_link", "new_monomer", "removed_link", "removed_monomer",
+              "moved_source_monomer", "not_moved_source_monomer"):
+        getattr(lib, n).restype = None
+    return lib
+
+
+def _bag_setup(lib, oracle, nt, nx, bc, wmm, m, mu, f):
+    C.c_double.in_dll(lib, "m").value = m
+    C.c_double.in_dll(lib, "mu").value = mu
+    assert lib.thirring_bag_init(nt, nx, bc, wmm) == 0
+    field = Rows(nt, nx, np.int32, f)
+    C.c_void_p.in_dll(lib, "field").value = field.ptr.value
+    B = oracle.bag_lattice(nt, nx, bc=bc, with_mass_monomers=wmm, m=m, mu=mu)
+    return field, B
+
+
+def test_bag_operator_signatures(bag_lib, oracle):
+    nt, nx = 8, 6
+    rng = np.random.default_rng(21)
+    f = np.zeros(nt * nx, np.int32)
+    f[[3, 4]] = [2 + 1, 4 + 1]
+    f[[17, 29]] = 1
+    field, B = _bag_setup(bag_lib, oracle, nt, nx, 0, 0, 0.3, 0.2, f)
+    try:
+        psi = rng.normal(size=nt * nx)
+        pin, out = Rows(nt, nx, np.float64, psi), Rows(nt, nx, np.float64)
+        bag_lib.fM(out.ptr, pin.ptr)  # (out, in) as the reference
+        assert np.max(np.abs(out.natural() - oracle.fM(B, f, psi))) <= 1e-14
+        bag_lib.fM_transpose(out.ptr, pin.ptr)
+        assert np.max(np.abs(out.natural() - oracle.fM(B, f, psi, transpose=True))) <= 1e-14
+        bag_lib.cg_propagator(out.ptr, pin.ptr)
+        xo, it, st = oracle.cg_propagator(B, f, psi, tol_abs_rr=1e-30)
+        assert np.max(np.abs(out.natural() - xo)) <= 1e-10
+        for (t1, x1, t2, x2) in [(0, 0, 1, 0), (2, 3, 2, 4), (7, 5, 0, 5), (3, 0, 3, 5), (4, 4, 4, 4)]:
+            assert bag_lib.fM_index(t1, x1, t2, x2, 0.2) == oracle.fM_index(B, t1, x1, t2, x2, 0.2)
+        la, sg, _ = oracle.determinant_mu(B, f, 0.2)
+        assert abs(np.log(abs(bag_lib.determinant())) - la) <= 1e-10 * max(1.0, abs(la))
+        assert np.sign(bag_lib.determinant()) == sg
+        assert bag_lib.thirring_bag_last_status() == 0
+    finally:
+        C.c_void_p.in_dll(bag_lib, "field").value = None
+
+
+def test_bag_calc_dinv_signature(bag_lib, oracle):
+    nt, nx = 6, 6
+    field, B = _bag_setup(bag_lib, oracle, nt, nx, 0, 1, 0.0, 0.1, np.zeros(36, np.int32))
+    h = nt * nx // 2
+    buf = np.zeros(nt * nx * nt * nx)
+    g = C.c_void_p.in_dll(bag_lib, "Dinv")
+    g.value = buf.ctypes.data
+    try:
+        bag_lib.calc_Dinv()
+        assert np.max(np.abs(buf[:2 * h * h] - oracle.calc_Dinv(B))) <= 1e-12
+    finally:
+        g.value = None
+        C.c_void_p.in_dll(bag_lib, "field").value = None
+
+
+def test_bag_update_sequence(bag_lib, oracle):
+    """det_added_link / det_removed_link / det_moved_monomer and the accept
+    hooks, driven like fermionbag.c's add_link / remove_link / worm, against
+    full determinants of the oracle; the hooks keep field, n_monomers,
+    n_links and current_sign as the reference does."""
+    nt, nx = 6, 6
+    V = nt * nx
+    f = np.zeros(V, np.int32)
+    field, B = _bag_setup(bag_lib, oracle, nt, nx, 0, 0, 0.4, 0.0, f)
+    C.c_int.in_dll(bag_lib, "max_changes").value = 7
+    nmono = C.c_int.in_dll(bag_lib, "n_monomers")
+    nlinks = C.c_int.in_dll(bag_lib, "n_links")
+    nmono.value = nlinks.value = 0
+    want_mono = want_links = 0
+    sign = C.c_int.in_dll(bag_lib, "current_sign")
+    try:
+        bag_lib.update_background()
+        rng = np.random.default_rng(23)
+        tried = acc = 0
+
+        def det(ff):
+            la, sg, _ = oracle.determinant_mu(B, ff)
+            return la, sg
+
+        la_d, sg_d = det(np.zeros(V, np.int32))
+        for _ in range(200):
+            cur = field.natural().astype(np.int32)
+            s = int(rng.integers(V))
+            t, x = divmod(s, nx)
+            nu = int(rng.integers(2))
+            t2, x2 = ((t + 1) % nt, x) if nu == 0 else (t, (x + 1) % nx)
+            s2 = t2 * nx + x2
+            la0, sg0 = det(cur)
+            new = cur.copy()
+            if cur[s] == 0 and cur[s2] == 0:
+                kind = int(rng.integers(2))
+                r = bag_lib.det_added_link(t, x, t2, x2)
+                new[s], new[s2] = (2 + nu, (nu + 2) % 4 + 2) if kind == 0 else (1, 1)
+            elif cur[s] == 2 + nu and cur[s2] == (nu + 2) % 4 + 2:
+                kind = 2
+                r = bag_lib.det_removed_link(t, x, t2, x2)
+                new[s] = new[s2] = 0
+            elif cur[s] == 1 and cur[s2] == 0 and (t + x) % 2 == (t2 + x2) % 2:
+                kind = 3
+                r = bag_lib.det_moved_monomer(t, x, t2, x2)
+                new[s], new[s2] = 0, 6
+            else:
+                continue
+            la1, sg1 = det(new)
+            want = np.exp(la1 - la0)
+            assert abs(r - want) <= 1e-9 * want
+            tried += 1
+            if rng.random() < 0.6:
+                if kind == 0:
+                    bag_lib.new_link(t, x, nu)  # n_links is the caller's (fermionbag.c)
+                elif kind == 1:
+                    bag_lib.new_monomer(t, x, t2, x2)
+                    want_mono += 2
+                elif kind == 2:
+                    bag_lib.removed_link(t, x, nu)
+                    want_links -= 1
+                else:
+                    bag_lib.moved_source_monomer(t, x, t2, x2)
+                acc += 1
+                assert np.array_equal(field.natural(), new)
+                assert sign.value == sg1 * sg_d
+        assert bag_lib.thirring_bag_last_status() == 0
+        assert tried > 40 and acc > 20
+        assert nmono.value == want_mono and nlinks.value == want_links
+    finally:
+        C.c_void_p.in_dll(bag_lib, "field").value = None
+        C.c_int.in_dll(bag_lib, "max_changes").value = 0

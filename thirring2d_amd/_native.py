@@ -122,6 +122,8 @@ SIGNATURES = {
                                         C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P),
                                         C.POINTER(D), U64, C.POINTER(U64),
                                         C.POINTER(HmcResult)]),
+    "th2d_fermion_matrix": (I, [P, C.POINTER(Geom), P, P]),
+    "th2d_zdet": (I, [P, I, P, C.POINTER(D), C.POINTER(D)]),
     # th2d_bag.h
     "th2d_bag_apply": (I, [P, C.POINTER(BagGeom), P, P, P, I, I]),
     "th2d_bag_cg": (I, [P, C.POINTER(BagGeom), P, P, P, I, D, I, C.POINTER(I), C.POINTER(I)]),
@@ -143,6 +145,8 @@ SIGNATURES = {
     "th2d_bag_calc_dinv": (I, [P, C.POINTER(BagGeom), P]),
     "th2d_bag_small_weights": (I, [P, C.POINTER(BagGeom), P, P, L, P]),
     "th2d_bag_brute_enumerate": (I, [P, C.POINTER(BagGeom), P, P, C.POINTER(L)]),
+    "th2d_bag_matrix_element": (I, [C.POINTER(BagGeom), I, I, I, I, D, C.POINTER(D)]),
+    "th2d_bag_determinant": (I, [P, C.POINTER(BagGeom), P, C.POINTER(D), C.POINTER(I)]),
 }
 
 _lib = None

@@ -1,4 +1,6 @@
-"""Build libth2d.so in-tree for gfx950 with hipcc (no JIT, no cache dir).
+"""Build libth2d.so in-tree for gfx950 with hipcc (no JIT, no cache dir),
+and the two C libraries with the reference's function signatures on top
+of it (libthirring_hmc.so, libthirring_bag.so).
 
     python -m thirring2d_amd.build
 """
@@ -12,6 +14,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libth2d.so")
+COMPAT = os.path.join(HERE, "compat")
+COMPAT_LIBS = {"libthirring_hmc.so": "thirring_hmc.c", "libthirring_bag.so": "thirring_bag.c"}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # Packed FP32 instructions are disabled for every HIP source.
@@ -42,6 +46,13 @@ def build(verbose: bool = False) -> str:
     cmd = [HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + sources() + objs
     _run(cmd, verbose)
     os.replace(tmp, OUT)
+    for lib, src in COMPAT_LIBS.items():
+        out = os.path.join(HERE, lib)
+        cmd = ["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-Wall", "-Wextra",
+               "-o", out + ".tmp", os.path.join(COMPAT, src), "-L" + HERE, "-lth2d",
+               "-Wl,-rpath,$ORIGIN", "-lm"]
+        _run(cmd, verbose)
+        os.replace(out + ".tmp", out)
     return OUT
 
 

@@ -1,0 +1,320 @@
+/*
+ * thirring_bag.c -- the reference's fermion-bag functions with their own
+ * signatures over libth2d.  Every matrix operation, solve, LU and rank-k
+ * update runs on the GPU; this file marshals the jagged field and vectors
+ * and keeps the reference's bookkeeping (field values, counters, sign).
+ */
+#include "../../include/thirring_bag.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/th2d.h"
+#include "../../include/th2d_bag.h"
+
+#define SOURCE_MONOMER 6 /* 2 + NDIRS, Thirring.h:39 */
+#define CG_ACCURACY 1e-30 /* Thirring.h:42 */
+#define CG_MAX_ITER 10000 /* Thirring.h:43 */
+
+/* ---- globals of fermionbag.c; the program's own definitions win ------- */
+__attribute__((weak)) int **field = NULL;
+__attribute__((weak)) double m = 0.0;
+__attribute__((weak)) double mu = 0.0;
+__attribute__((weak)) double *Dinv = NULL;
+__attribute__((weak)) int n_monomers = 0;
+__attribute__((weak)) int n_links = 0;
+__attribute__((weak)) int current_sign = 1;
+__attribute__((weak)) int max_changes = 0;
+__attribute__((weak)) void update_linklists(void) {}
+
+static struct {
+  th2d_ctx *ctx;
+  th2d_bag_geom geo;
+  int nt, nx, V;
+  th2d_bag *bag;
+  double bag_m, bag_mu;
+  int *dfield;
+  double *dv0, *dv1, *dD;
+  int *hf;
+  double *hv;
+  int changes;
+  int status;
+} S;
+
+static int fail(int rc)
+{
+  if (rc != TH2D_OK && S.status == TH2D_OK) S.status = rc;
+  return rc;
+}
+
+int thirring_bag_init(int nt, int nx, int bc, int with_mass_monomers)
+{
+  thirring_bag_finalize();
+  if (nt < 4 || nx < 4 || (nt & 1) || (nx & 1) || (bc != 0 && bc != 1)) return TH2D_EARG;
+  int rc = th2d_ctx_create(&S.ctx, NULL);
+  if (rc) return rc;
+  S.nt = nt;
+  S.nx = nx;
+  S.V = nt * nx;
+  S.geo = (th2d_bag_geom){nt, nx, bc, with_mass_monomers, m, mu};
+  size_t vb = sizeof(double) * (size_t)S.V;
+  if ((rc = th2d_malloc((void **)&S.dfield, sizeof(int) * S.V))) return rc;
+  if ((rc = th2d_malloc((void **)&S.dv0, vb))) return rc;
+  if ((rc = th2d_malloc((void **)&S.dv1, vb))) return rc;
+  if ((rc = th2d_malloc((void **)&S.dD, sizeof(double) * (size_t)S.V * S.V / 2))) return rc;
+  S.hf = malloc(sizeof(int) * S.V);
+  S.hv = malloc(vb);
+  if (!S.hf || !S.hv) return TH2D_ENOMEM;
+  S.status = TH2D_OK;
+  return TH2D_OK;
+}
+
+void thirring_bag_finalize(void)
+{
+  if (!S.ctx) return;
+  if (S.bag) th2d_bag_destroy(S.bag);
+  void *dev[] = {S.dfield, S.dv0, S.dv1, S.dD};
+  for (int k = 0; k < 4; k++)
+    if (dev[k]) th2d_free(dev[k]);
+  free(S.hf);
+  free(S.hv);
+  th2d_ctx_destroy(S.ctx);
+  memset(&S, 0, sizeof(S));
+}
+
+int thirring_bag_last_status(void) { return S.status; }
+
+static int ready(void)
+{
+  if (!S.ctx) return 0;
+  S.geo.m = m;
+  S.geo.mu = mu;
+  S.status = TH2D_OK;
+  return 1;
+}
+
+static void pack_field(void)
+{
+  for (int t = 0; t < S.nt; t++)
+    for (int x = 0; x < S.nx; x++) S.hf[t * S.nx + x] = field ? field[t][x] : 0;
+}
+
+static int upload_field(void)
+{
+  pack_field();
+  return fail(th2d_h2d(S.ctx, S.dfield, S.hf, sizeof(int) * S.V));
+}
+
+static int load_vec(double **v, double *d)
+{
+  for (int t = 0; t < S.nt; t++)
+    for (int x = 0; x < S.nx; x++) S.hv[t * S.nx + x] = v[t][x];
+  return fail(th2d_h2d(S.ctx, d, S.hv, sizeof(double) * S.V));
+}
+
+static int store_vec(const double *d, double **v)
+{
+  int rc = fail(th2d_d2h(S.ctx, S.hv, d, sizeof(double) * S.V));
+  if (!rc)
+    for (int t = 0; t < S.nt; t++)
+      for (int x = 0; x < S.nx; x++) v[t][x] = S.hv[t * S.nx + x];
+  return rc;
+}
+
+/* neighbour in direction dir (0 t up, 1 x up, 2 t down, 3 x down) */
+static void nb(int t, int x, int dir, int *t2, int *x2)
+{
+  *t2 = dir == 0 ? (t + 1) % S.nt : dir == 2 ? (t - 1 + S.nt) % S.nt : t;
+  *x2 = dir == 1 ? (x + 1) % S.nx : dir == 3 ? (x - 1 + S.nx) % S.nx : x;
+}
+
+/* ---- fermion_matrix.c -------------------------------------------------- */
+double fM_index(int t1, int x1, int t2, int x2, double mu_)
+{
+  if (!ready()) return NAN;
+  double v = NAN;
+  fail(th2d_bag_matrix_element(&S.geo, t1, x1, t2, x2, mu_, &v));
+  return v;
+}
+
+void calc_Dinv(void)
+{
+  if (!ready()) return;
+  if (fail(th2d_bag_calc_dinv(S.ctx, &S.geo, S.dD))) return;
+  if (!Dinv) Dinv = malloc(sizeof(double) * (size_t)S.V * S.V);
+  if (Dinv) fail(th2d_d2h(S.ctx, Dinv, S.dD, sizeof(double) * (size_t)S.V * S.V / 2));
+}
+
+/* ---- vec_ops.c ---------------------------------------------------------- */
+static void apply(double **chi, double **psi, int transpose)
+{
+  if (!ready() || upload_field() || load_vec(psi, S.dv0)) return;
+  if (!fail(th2d_bag_apply(S.ctx, &S.geo, S.dfield, S.dv1, S.dv0, transpose, 1))) store_vec(S.dv1, chi);
+}
+
+void fM(double **chi, double **psi) { apply(chi, psi, 0); }
+
+void fM_transpose(double **chi, double **psi) { apply(chi, psi, 1); }
+
+void cg_propagator(double **propagator, double **source)
+{
+  if (!ready() || upload_field() || load_vec(source, S.dv0)) return;
+  int it = 0, st = 0;
+  int rc = th2d_bag_cg(S.ctx, &S.geo, S.dfield, S.dv1, S.dv0, 1, CG_ACCURACY, CG_MAX_ITER, &it, &st);
+  /* divergence fills the result with 1e50 as vec_ops.c:292-296 does */
+  if (rc != TH2D_OK && rc != TH2D_EBREAKDOWN && rc != TH2D_ENOCONV) fail(rc);
+  else if (rc != TH2D_OK) S.status = rc;
+  store_vec(S.dv1, propagator);
+}
+
+/* ---- determinants --------------------------------------------------------- */
+double determinant_mu(double mu_)
+{
+  if (!ready()) return NAN;
+  pack_field();
+  th2d_bag_geom g = S.geo;
+  g.mu = mu_;
+  double la = 0.0;
+  int sg = 0;
+  if (fail(th2d_bag_determinant(S.ctx, &g, S.hf, &la, &sg))) return NAN;
+  return sg == 0 ? 0.0 : sg * exp(la);
+}
+
+double determinant(void) { return determinant_mu(mu); }
+
+static int set_sign(void)
+{
+  double la, lad;
+  int sg, sgd;
+  int rc = th2d_bag_logdet(S.bag, &la, &sg, &lad, &sgd);
+  if (!rc) current_sign = sg * sgd; /* sign of det_e det_o = det(D_FF) / det(D) */
+  return fail(rc);
+}
+
+void update_background(void)
+{
+  if (!ready()) return;
+  if (S.bag && (S.bag_m != m || S.bag_mu != mu)) {
+    th2d_bag_destroy(S.bag);
+    S.bag = NULL;
+  }
+  if (!S.bag) {
+    if (fail(th2d_bag_create(S.ctx, &S.geo, &S.bag))) return;
+    S.bag_m = m;
+    S.bag_mu = mu;
+  }
+  pack_field();
+  if (fail(th2d_bag_set_field(S.bag, S.hf))) return;
+  S.changes = 0;
+  set_sign();
+}
+
+static int state(void)
+{
+  if (!ready()) return 0;
+  if (!S.bag) update_background();
+  return S.bag != NULL;
+}
+
+static double ratio(const int *occ, int no, const int *fr, int nf)
+{
+  double r = 0.0;
+  int rc = th2d_bag_ratio(S.bag, occ, no, fr, nf, &r);
+  if (rc == TH2D_ESINGULAR) return 0.0;
+  if (fail(rc)) return NAN;
+  return fabs(r);
+}
+
+double det_added_link(int t, int x, int t2, int x2)
+{
+  if (!state()) return NAN;
+  int occ[2] = {t * S.nx + x, t2 * S.nx + x2};
+  return ratio(occ, 2, NULL, 0);
+}
+
+double det_removed_link(int t, int x, int t2, int x2)
+{
+  if (!state()) return NAN;
+  int fr[2] = {t * S.nx + x, t2 * S.nx + x2};
+  return ratio(NULL, 0, fr, 2);
+}
+
+double det_moved_monomer(int t, int x, int t2, int x2)
+{
+  if (!state()) return NAN;
+  if (((t + x) & 1) != ((t2 + x2) & 1)) return 0.0;
+  int fr[1] = {t * S.nx + x}, occ[1] = {t2 * S.nx + x2};
+  return ratio(occ, 1, fr, 1);
+}
+
+/* commit a change to the device state, then the reference's bookkeeping */
+static void commit(const int *occ, const int *vals, int no, const int *fr, int nf)
+{
+  if (fail(th2d_bag_accept(S.bag, occ, no, vals, fr, nf))) return;
+  set_sign();
+  update_linklists();
+  if (max_changes > 0 && ++S.changes >= max_changes) update_background();
+}
+
+void new_link(int t, int x, int nu)
+{
+  if (!state()) return;
+  int t2, x2;
+  nb(t, x, nu, &t2, &x2);
+  field[t][x] = nu + 2;
+  field[t2][x2] = (nu + 2) % 4 + 2; /* opp_dir(nu) + 2 */
+  int occ[2] = {t * S.nx + x, t2 * S.nx + x2}, vals[2] = {field[t][x], field[t2][x2]};
+  commit(occ, vals, 2, NULL, 0);
+}
+
+void new_monomer(int t1, int x1, int t2, int x2)
+{
+  if (!state()) return;
+  n_monomers += 2;
+  field[t1][x1] = 1;
+  field[t2][x2] = 1;
+  int occ[2] = {t1 * S.nx + x1, t2 * S.nx + x2}, vals[2] = {1, 1};
+  commit(occ, vals, 2, NULL, 0);
+}
+
+void removed_link(int t, int x, int nu)
+{
+  if (!state()) return;
+  n_links -= 1;
+  int t2, x2;
+  nb(t, x, nu, &t2, &x2);
+  field[t][x] = 0;
+  field[t2][x2] = 0;
+  int fr[2] = {t * S.nx + x, t2 * S.nx + x2};
+  commit(NULL, NULL, 0, fr, 2);
+}
+
+void removed_monomer(int t1, int x1, int nu)
+{
+  if (!state()) return;
+  n_monomers -= 2;
+  int t2, x2;
+  nb(t1, x1, nu, &t2, &x2);
+  field[t1][x1] = 0;
+  field[t2][x2] = 0;
+  int fr[2] = {t1 * S.nx + x1, t2 * S.nx + x2};
+  commit(NULL, NULL, 0, fr, 2);
+}
+
+void moved_source_monomer(int t2, int x2, int t4, int x4)
+{
+  if (!state()) return;
+  field[t2][x2] = 0;
+  field[t4][x4] = SOURCE_MONOMER;
+  int fr[1] = {t2 * S.nx + x2}, occ[1] = {t4 * S.nx + x4}, vals[1] = {SOURCE_MONOMER};
+  commit(occ, vals, 1, fr, 1);
+}
+
+void not_moved_source_monomer(int t2, int x2, int t4, int x4)
+{
+  (void)t2;
+  (void)x2;
+  (void)t4;
+  (void)x4;
+}

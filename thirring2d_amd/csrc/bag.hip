@@ -1000,3 +1000,65 @@ extern "C" int th2d_bag_brute_enumerate(th2d_ctx *c, const th2d_bag_geom *g, dou
   (void)hipFree(cfg);
   return rc;
 }
+
+// Matrix element of fM_index (fermion_matrix.c:28-64, 152-182) at the
+// given mu, evaluated on the host by the function the kernels use.
+extern "C" int th2d_bag_matrix_element(const th2d_bag_geom *g, int t1, int x1, int t2, int x2,
+                                       double mu, double *out)
+{
+  TH2D_TRY(check_bag_geom(g));
+  if (!out || t1 < 0 || t2 < 0 || x1 < 0 || x2 < 0 || t1 >= g->nt || t2 >= g->nt ||
+      x1 >= g->nx || x2 >= g->nx)
+    return TH2D_EARG;
+  th2d_bag_geom h = *g;
+  h.mu = mu;
+  const BagGeo G = bag_geo(&h);
+  *out = bag_elem(G, t1 * g->nx + x1, t2 * g->nx + x2);
+  return TH2D_OK;
+}
+
+// det of the matrix over the free sites of a configuration
+// (determinant_mu, full_determinant.c:144-186) by LU on the device.
+extern "C" int th2d_bag_determinant(th2d_ctx *c, const th2d_bag_geom *g, const int *field_host,
+                                    double *logabs, int *sign)
+{
+  TH2D_TRY(check_bag_geom(g));
+  if (!c || !field_host || !logabs || !sign) return TH2D_EARG;
+  th2d_bag *bg = new th2d_bag();
+  bg->ctx = c;
+  bg->geom = *g;
+  bg->G = bag_geo(g);
+  bg->V = g->nt * g->nx;
+  const long V = bg->V;
+  bool ok = hipMalloc(&bg->work, sizeof(double) * V * V) == hipSuccess &&
+            hipMalloc(&bg->field, sizeof(int) * V) == hipSuccess &&
+            hipMalloc(&bg->dgeom, sizeof(th2d_bag_geom)) == hipSuccess &&
+            hipMalloc(&bg->boff, sizeof(long) * 2) == hipSuccess &&
+            hipMalloc(&bg->bn, sizeof(int) * 2) == hipSuccess &&
+            hipMalloc(&bg->piv, sizeof(int) * V) == hipSuccess &&
+            hipMalloc(&bg->info, sizeof(int) * 2) == hipSuccess &&
+            hipMalloc(&bg->dlog, sizeof(double) * 2) == hipSuccess &&
+            hipMalloc(&bg->dsign, sizeof(int) * 2) == hipSuccess;
+  int rc = ok ? TH2D_OK : TH2D_ENOMEM;
+  if (rc == TH2D_OK) {
+    long zero[2] = {0, 0};
+    int nn[2] = {bg->V, bg->V};
+    TH2D_HIP(hipMemcpyAsync(bg->boff, zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
+    TH2D_HIP(hipMemcpyAsync(bg->bn, nn, sizeof(nn), hipMemcpyHostToDevice, c->stream));
+    TH2D_HIP(hipMemcpyAsync(bg->dgeom, g, sizeof(*g), hipMemcpyHostToDevice, c->stream));
+    TH2D_HIP(hipMemcpyAsync(bg->field, field_host, sizeof(int) * V, hipMemcpyHostToDevice,
+                            c->stream));
+    th2d_batch bt{1, bg->V, bg->boff, bg->bn, bg->boff};
+    rc = th2d_bag_build_masked(c, bg->dgeom, bg->field, bg->boff, &bt, bg->work, TH2D_PREC_F64);
+    if (rc == TH2D_OK) rc = th2d_lu_batched(c, &bt, bg->work, TH2D_PREC_F64, bg->piv, bg->info);
+    if (rc == TH2D_OK)
+      rc = th2d_lu_logdet(c, &bt, bg->work, TH2D_PREC_F64, bg->piv, bg->info, bg->dlog, bg->dsign);
+    if (rc == TH2D_OK) {
+      TH2D_HIP(hipMemcpyAsync(logabs, bg->dlog, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      TH2D_HIP(hipMemcpyAsync(sign, bg->dsign, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      TH2D_HIP(hipStreamSynchronize(c->stream));
+    }
+  }
+  th2d_bag_destroy(bg);
+  return rc;
+}
