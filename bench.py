@@ -28,6 +28,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# measured HBM bytes per k_cg_step launch (profiles/k_cg_step_hmc256_32chains_mixed.md)
+TRAFFIC_BYTES = {(256, 32): 199.6e6}
 
 
 def parse():
@@ -197,7 +199,10 @@ def main():
         "acceptance": float(np.mean([r[1] for r in res_mx])),
         "dH_median": float(np.median([r[0] for r in res_mx])),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     # HBM bytes per launch of this kernel at this workload, from the
+                     # FETCH_SIZE / WRITE_SIZE passes in profiles/ (copy-calibrated)
+                     "traffic": TRAFFIC_BYTES.get((size, nch)),
                      "kernel": "k_cg_step<float> (CG iteration, 120 B/site algorithmic)",
                      "launches": kn, "avg_launch_ms": kms / max(kn, 1)},
     }

@@ -160,7 +160,8 @@ int th2d_set_cg_max_blocks(th2d_ctx *ctx, int n);
 /* Tile shape used by the last solve on this context. */
 int th2d_cg_last_tile(th2d_ctx *ctx, int *tt, int *tx);
 /* Solver launches enqueued between two reads of the stop flags
- * (0 = the previous solve's iteration count, then 32). */
+ * (0 = the iterations the same restart pass took in the previous solve,
+ * then 8 at a time). */
 int th2d_set_cg_chunk(th2d_ctx *ctx, int n);
 int th2d_set_dslash_variant(int variant);
 

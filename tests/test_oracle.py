@@ -267,3 +267,37 @@ def test_brute_table_matches_reference_run(oracle):
     assert f"{Z[1, 0]:g}" == "9.3976"
     assert f"{Z[2, 2]:g}" == "629.076"
     assert int((Z != 0).sum()) == 45
+
+
+def test_bag_matrix_is_hmc_matrix_at_zero_field(oracle):
+    """fM_index on an antisymmetric lattice (fermion_matrix.c:28-64) is the
+    HMC operator of hmc.c:269-310 at A = 0, whose log|det| the free-field
+    closed form pins (test_free_field_logdet)."""
+    nt, nx, m, mu = 6, 8, 0.3, 0.2
+    B = oracle.bag_lattice(nt, nx, bc=oracle.BC_ANTISYMMETRIC, m=m, mu=mu)
+    V = nt * nx
+    D = np.array([[oracle.fM_index(B, i // nx, i % nx, j // nx, j % nx) for j in range(V)]
+                  for i in range(V)])
+    L = oracle.lattice(nt, nx, m=m, mu=mu)
+    M = oracle.fermion_matrix(L, np.zeros((V, 2)))
+    assert np.max(np.abs(M.imag)) == 0.0
+    assert np.max(np.abs(D - M.real)) <= 1e-15
+
+
+def test_puregauge_sweep_samples_link_distribution(oracle):
+    """Metropolis with uniform proposals samples exp(beta cos A) per link
+    (hmc.c:82-93), beta = Nf/g: <cos A> = I1(beta)/I0(beta)."""
+    from scipy.special import i0, i1
+    nt, nx, g = 8, 8, 0.8
+    L = oracle.lattice(nt, nx, g=g)
+    rng = oracle.Rng(77)
+    A = np.zeros((nt * nx, 2))
+    acc = []
+    for k in range(600):
+        A = oracle.puregauge_sweep(L, rng, A)
+        if k >= 100:
+            acc.append(np.mean(np.cos(A)))
+    beta = 2.0 / g
+    want = i1(beta) / i0(beta)
+    err = np.std(acc) / np.sqrt(len(acc) / 5.0)  # sweeps are correlated
+    assert abs(np.mean(acc) - want) <= 5 * err + 1e-3

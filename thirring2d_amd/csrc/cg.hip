@@ -498,7 +498,8 @@ struct InnerResult {
 // Batched inner solve: A_c y_c = rhs_c from y_c = 0, stop r.r < stop_c.
 // The solutions are the internal x vectors, returned in xout[c].
 template <typename T>
-int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T> *const *rhs,
+int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
+                const c2<T> *const *rhs,
                 const c2<T> *const *K, const double *stop_rr, const int *max_iter, c2<T> **xout,
                 InnerResult *res)
 {
@@ -579,8 +580,10 @@ int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T>
   TH2D_HIP(hipGetLastError());
   std::vector<CgResult> h(nch);
   long launched = 0;
-  int &last = ctx->cg_last_iters[sizeof(T) == 8 ? 1 : 0];
-  int chunk = ctx->cg_chunk > 0 ? ctx->cg_chunk : std::max(16, last + 4);
+  // launches per stop-flag read: the iteration count this restart pass took
+  // in the previous solve (solves of one trajectory are alike), then 8
+  int &last = ctx->cg_pass_iters[sizeof(T) == 8 ? 1 : 0][std::min(pass, 7)];
+  int chunk = ctx->cg_chunk > 0 ? ctx->cg_chunk : (last > 0 ? last + 1 : 32);
   int most = 0;
   for (int c = 0; c < nch; c++) most = std::max(most, max_iter[c]);
   for (;;) {
@@ -600,7 +603,7 @@ int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T>
         if (!h[c].done) h[c].status = TH2D_ENOCONV;
       break;
     }
-    chunk = ctx->cg_chunk > 0 ? ctx->cg_chunk : 32;
+    chunk = ctx->cg_chunk > 0 ? ctx->cg_chunk : 8;
   }
   int itmax = 0;
   for (int c = 0; c < nch; c++) itmax = std::max(itmax, h[c].iters);
@@ -770,7 +773,7 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
         rh[k] = R32 + voff[idx[k]];
         kk[k] = (const float2 *)K32[idx[k]];
       }
-      TH2D_TRY(inner_solve<float>(ctx, ctx->cg32, na, g2.data(), rh.data(), kk.data(), stop.data(),
+      TH2D_TRY(inner_solve<float>(ctx, ctx->cg32, outer, na, g2.data(), rh.data(), kk.data(), stop.data(),
                                   mi.data(), xo.data(), ir.data()));
       for (int k = 0; k < na; k++) {
         int c = idx[k];
@@ -785,7 +788,7 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
         rh[k] = (const double2 *)(outer == 0 ? b[c] : TMP + 2 * voff[c]);
         kk[k] = (const double2 *)K[c];
       }
-      TH2D_TRY(inner_solve<double>(ctx, ctx->cg64, na, g2.data(), rh.data(), kk.data(),
+      TH2D_TRY(inner_solve<double>(ctx, ctx->cg64, outer, na, g2.data(), rh.data(), kk.data(),
                                    stop.data(), mi.data(), xo.data(), ir.data()));
       for (int k = 0; k < na; k++) {
         int c = idx[k];

@@ -51,9 +51,9 @@ struct th2d_ctx {
   // tile configuration of the persistent solver (0 = autotuned default)
   int tile_t = 0, tile_x = 0;
   int cg_max_blocks = 0;  // 0 = one workgroup per tile
-  int cg_chunk = 0;       // step launches per stop-flag check (0 = 32, doubling)
+  int cg_chunk = 0;       // step launches per stop-flag check (0 = from history)
   long kiters = 0;
-  int cg_last_iters[2] = {0, 0};  // FP32, FP64 inner solves
+  int cg_pass_iters[2][8] = {};   // iterations per restart pass (FP32, FP64)
   int cg_last_tile[2] = {0, 0};   // tile shape of the last inner solve
 };
 
