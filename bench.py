@@ -43,6 +43,12 @@ def parse():
     ap.add_argument("--no-dslash", action="store_true")
     ap.add_argument("--only-headline", action="store_true",
                     help="time the batched mixed-precision leg only (for profiling)")
+    ap.add_argument("--c4-steps", type=int, default=3,
+                    help="trajectories per chain of the 256-chain uneven batch (0 = skip)")
+    ap.add_argument("--c5-proposals", type=int, default=150,
+                    help="bag proposals per matrix in the dense leg (0 = skip the leg)")
+    ap.add_argument("--c1-steps", type=int, default=40,
+                    help="16^2 FP64 trajectories (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB five times and exit (scale of "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -60,31 +66,30 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 5}), flush=True)
 
 
-def run_hmc(H, N, size, precision, steps, warmup, chains=1, seed_links=1002):
-    """Time `steps` trajectories of `chains` independent chains (one batched
-    call per step).  Returns (seconds, per-trajectory records, kernel ms,
-    kernel launches, kernel bytes)."""
+def run_batch(H, N, lats, gs, precision, steps, warmup, nsteps=10):
+    """Time `steps` trajectories of every chain of `lats` (one batched call
+    per step).  Returns (seconds, per-trajectory records, kernel ms, kernel
+    launches, kernel bytes)."""
     import ctypes as C
-    lats = []
-    for c in range(chains):
-        lat = H.StaggeredLattice(size, size, m=0.05, g=0.5)
-        lat.random_links(seed_links + c)
-        lats.append(lat)
     batch = H.ChainBatch(lats)
+    for l, g in zip(lats, gs):
+        l.g = g
     params = H.cg_params(tol_rel=1e-10, precision=precision, max_iter=200000)
     L = N.lib()
+    nch = len(lats)
     ctr = 0
     for _ in range(warmup):
-        batch.update_gauge(nsteps=10, traj_len=1.0, params=params, seed=4242,
-                           counters=[ctr] * chains)
+        batch.update_gauge(nsteps=nsteps, traj_len=1.0, params=params, seed=4242,
+                           counters=[ctr] * nch)
         ctr += 1
     L.th2d_kernel_timing(batch.ctx.ptr, 1)
     batch.ctx.sync()
     t0 = time.perf_counter()
     res = []
     for _ in range(steps):
-        rr = batch.update_gauge(nsteps=10, traj_len=1.0, params=params, seed=4242,
-                                counters=[ctr] * chains, u_accept=list(np.random.default_rng(ctr).random(chains)))
+        u = list(np.random.default_rng(ctr).random(nch))
+        rr = batch.update_gauge(nsteps=nsteps, traj_len=1.0, params=params, seed=4242,
+                                counters=[ctr] * nch, u_accept=u)
         ctr += 1
         res += [(r.dS, r.accepted, r.cg_iterations, r.cg_solves, r.max_true_rel_resid, r.status)
                 for r in rr]
@@ -94,6 +99,132 @@ def run_hmc(H, N, size, precision, steps, warmup, chains=1, seed_links=1002):
     L.th2d_kernel_time(batch.ctx.ptr, C.byref(ms), C.byref(n), C.byref(by))
     L.th2d_kernel_timing(batch.ctx.ptr, 0)
     return dt, res, ms.value, n.value, by.value
+
+
+def run_hmc(H, N, size, precision, steps, warmup, chains=1, seed_links=1002, m=0.05, g=0.5):
+    lats = []
+    for c in range(chains):
+        lat = H.StaggeredLattice(size, size, m=m, g=g)
+        lat.random_links(seed_links + c)
+        lats.append(lat)
+    return run_batch(H, N, lats, [g] * chains, precision, steps, warmup)
+
+
+def c4_leg(H, N, steps):
+    """256 chains, L uniform on {32,48,64,96,128} (seed 4004), g ~ U[0.2,1],
+    m = 0.1, mixed CG to 1e-10, 10 x dt = 0.1, autotuned tiles."""
+    rng = np.random.default_rng(4004)
+    Ls = rng.choice([32, 48, 64, 96, 128], size=256)
+    gs = rng.uniform(0.2, 1.0, size=256)
+    lats = []
+    for c, (Lc, gc) in enumerate(zip(Ls, gs)):
+        lat = H.StaggeredLattice(int(Lc), int(Lc), m=0.1, g=float(gc))
+        lat.random_links(40040 + c)
+        lats.append(lat)
+    dt, res, kms, kn, kby = run_batch(H, N, lats, list(gs), N.PREC_MIXED, steps, 1)
+    sites = int(np.sum(Ls.astype(np.int64) ** 2))
+    return {"value": 256 * steps / dt, "unit": "traj/s", "chains": 256, "sites": sites,
+            "trajectories": 256 * steps, "sizes": "L in {32,48,64,96,128}, seed 4004",
+            "max_true_rel_resid": max(r[4] for r in res),
+            "cg_kernel_GBs": kby / (kms * 1e-3) / 1e9 if kms > 0 else 0.0}
+
+
+def c1_leg(H, N, steps):
+    """16x16 as the reference runs it: m = 0.1, g = 0.5, FP64, 10 x dt = 0.1."""
+    dt, res, *_ = run_hmc(H, N, 16, N.PREC_F64, steps, 2, chains=1, seed_links=1001, m=0.1)
+    return {"value": steps / dt, "unit": "traj/s", "trajectories": steps,
+            "reference_cpu_1core_O3": 23.4, "max_true_rel_resid": max(r[4] for r in res)}
+
+
+def _bag_field(rng, L, rho):
+    """Random legal links and monomer pairs up to occupied fraction rho."""
+    V = L * L
+    f = np.zeros(V, np.int32)
+    target = int(rho * V)
+    tries = 0
+    while np.count_nonzero(f) < target and tries < 20 * V:
+        tries += 1
+        s = int(rng.integers(V))
+        t, x = divmod(s, L)
+        d = int(rng.integers(2))
+        s2 = ((t + 1) % L) * L + x if d == 0 else t * L + (x + 1) % L
+        if f[s] or f[s2]:
+            continue
+        if rng.random() < 0.8:
+            f[s], f[s2] = 2 + d, 4 + d
+        else:
+            f[s] = f[s2] = 1
+    return f
+
+
+def c5_leg(N, proposals):
+    """64 masked bag matrices, L uniform on even {16..48} (seed 5005),
+    occupancy rho ~ U[0.1, 0.5], m = 0.1: batched LU log-det in FP64 and
+    FP32 on MFMA, then rank-1/rank-2 ratio+update proposals."""
+    from thirring2d_amd import bag as Bm
+    from thirring2d_amd.hmc import Context
+    import ctypes as C
+    rng = np.random.default_rng(5005)
+    Ls = rng.choice(np.arange(16, 49, 2), size=64)
+    lats = [Bm.BagLattice(int(L), int(L), m=0.1) for L in Ls]
+    fields = [_bag_field(rng, int(L), rng.uniform(0.1, 0.5)) for L in Ls]
+    ctx = Context.get()
+    lib = N.lib()
+    out = {"matrices": 64, "orders": [int(L) ** 2 for L in Ls][:4] + ["..."],
+           "max_order": int(max(Ls)) ** 2}
+    flops = float(sum((2.0 / 3.0) * float(L) ** 6 for L in Ls))
+    la = {}
+    for prec, peak, tag in ((N.PREC_F64, 78.6, "f64"), (N.PREC_F32, 157.3, "f32")):
+        db = Bm.build_masked_batch(lats, fields, prec)
+        db.lu()  # warm-up
+        db = Bm.build_masked_batch(lats, fields, prec)
+        ctx.sync()
+        lib.th2d_timer_start(ctx.ptr)
+        db.lu()
+        ms = C.c_float()
+        lib.th2d_timer_stop(ctx.ptr, C.byref(ms))
+        la[tag], sg = db.logdet()
+        tf = flops / (ms.value * 1e-3) / 1e12
+        out["lu_" + tag] = {"ms": ms.value, "TFLOP/s": tf, "peak": peak, "mfma_frac": tf / peak}
+    out["f32_logdet_max_abs_err_vs_f64"] = float(np.max(np.abs(la["f32"] - la["f64"])))
+    out["f32_logdet_max_rel_err_vs_f64"] = float(np.max(np.abs(la["f32"] - la["f64"]) /
+                                                        np.abs(la["f64"])))
+    # rank-1/rank-2 updates on the device-resident inverse of a subset
+    prng = np.random.default_rng(5006)
+    n_up = n_acc = 0
+    t_up = 0.0
+    for k in range(8):
+        L = int(Ls[k])
+        st = Bm.BagState(lats[k])
+        st.set_field(fields[k])
+        f = fields[k].copy()
+        t0 = time.perf_counter()
+        for _ in range(proposals):
+            s = int(prng.integers(L * L))
+            t, x = divmod(s, L)
+            d = int(prng.integers(2))
+            s2 = ((t + 1) % L) * L + x if d == 0 else t * L + (x + 1) % L
+            if f[s] == 0 and f[s2] == 0:
+                occ, vals, fr = [s, s2], [2 + d, 4 + d], []
+            elif f[s] == 2 + d and f[s2] == 4 + d:
+                occ, vals, fr = [], [], [s, s2]
+            elif f[s] == 1 and f[s2] == 0:
+                occ, vals, fr = [s2], [1], [s]
+            else:
+                continue
+            r = st.ratio(occ, fr)
+            n_up += 1
+            if abs(r) > 1e-8 and prng.random() < min(1.0, abs(r)):
+                st.accept(occ, vals, fr)
+                f[fr] = 0
+                f[occ] = vals
+                n_acc += 1
+        t_up += time.perf_counter() - t0
+        del st
+    out["woodbury"] = {"proposals": n_up, "accepted": n_acc, "seconds": t_up,
+                       "proposals_per_s": n_up / t_up if t_up > 0 else 0.0,
+                       "matrices": 8}
+    return out
 
 
 def dslash_bw(H, N, size, nrhs, precision, reps=50):
@@ -215,6 +346,13 @@ def main():
             ms, gbs = dslash_bw(H, N, sz, nrhs, prec)
             d[tag] = {"ms": ms, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS}
         out["dslash"] = d
+    if args.c1_steps:
+        out["c1_16x16_fp64"] = c1_leg(H, N, args.c1_steps)
+    if args.c4_steps:
+        N.lib().th2d_set_cg_autotune(H.Context.get().ptr, 1)
+        out["c4_256_chains_uneven"] = c4_leg(H, N, args.c4_steps)
+    if args.c5_proposals:
+        out["c5_bag_dense"] = c5_leg(N, args.c5_proposals)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(its_164, size)
     print(json.dumps(out), flush=True)

@@ -157,6 +157,11 @@ int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *geoms, double
  * variant (0 direct, 1 LDS-tiled). */
 int th2d_set_cg_tile(th2d_ctx *ctx, int tt, int tx);
 int th2d_set_cg_max_blocks(th2d_ctx *ctx, int n);
+/* Measuring autotuner (0 off, default; 1 on): the first solve of a batch
+ * signature (its distinct lattice shapes and size class) times a few
+ * iterations with every tile shape that fits and keeps the fastest for
+ * this context; a tile set with th2d_set_cg_tile takes precedence. */
+int th2d_set_cg_autotune(th2d_ctx *ctx, int on);
 /* Tile shape used by the last solve on this context. */
 int th2d_cg_last_tile(th2d_ctx *ctx, int *tt, int *tx);
 /* Solver launches enqueued between two reads of the stop flags

@@ -112,6 +112,7 @@ SIGNATURES = {
     "th2d_puregauge_sweep": (I, [P, C.POINTER(Geom), P, P, D, I]),
     "th2d_set_cg_tile": (I, [P, I, I]),
     "th2d_cg_last_tile": (I, [P, C.POINTER(I), C.POINTER(I)]),
+    "th2d_set_cg_autotune": (I, [P, I]),
     "th2d_set_dslash_variant": (I, [I]),
     "th2d_set_cg_max_blocks": (I, [P, I]),
     "th2d_set_cg_chunk": (I, [P, I]),

@@ -112,16 +112,22 @@ class BagLattice:
 class DenseBatch:
     """Square matrices of uneven orders packed into one device buffer."""
 
-    def __init__(self, mats, precision=N.PREC_F64):
+    def __init__(self, mats=None, precision=N.PREC_F64, orders=None):
+        """From host matrices, or uninitialised device storage for the
+        given orders."""
         self.ctx = Context.get()
         self.prec = precision
         self.dtype = np.float64 if precision == N.PREC_F64 else np.float32
-        self.ns = [int(np.asarray(m).shape[0]) for m in mats]
+        self.ns = ([int(n) for n in orders] if mats is None
+                   else [int(np.asarray(m).shape[0]) for m in mats])
         off = np.cumsum([0] + [n * n for n in self.ns])[:-1].astype(np.int64)
         poff = np.cumsum([0] + self.ns)[:-1].astype(np.int64)
         self.off, self.poff = off, poff
-        flat = np.concatenate([np.asarray(m, dtype=self.dtype).ravel() for m in mats])
-        self.A = DeviceArray.from_host(flat)
+        if mats is None:
+            self.A = DeviceArray(np.dtype(self.dtype).itemsize * sum(n * n for n in self.ns))
+        else:
+            flat = np.concatenate([np.asarray(m, dtype=self.dtype).ravel() for m in mats])
+            self.A = DeviceArray.from_host(flat)
         self._off = DeviceArray.from_host(off)
         self._poff = DeviceArray.from_host(poff)
         self._n = DeviceArray.from_host(np.array(self.ns, dtype=np.int32))
@@ -204,3 +210,23 @@ class BagState:
                                         v.ctypes.data_as(C.c_void_p),
                                         f.ctypes.data_as(C.c_void_p), len(f)),
                 "th2d_bag_accept")
+
+
+def build_masked_batch(lattices, fields, precision=N.PREC_F64) -> DenseBatch:
+    """Dense masked matrices (identity rows/columns at occupied sites,
+    fM_index elements between free sites; vec_ops.c:107-131) of several
+    configurations on lattices of any sizes, built on the device in one
+    launch (th2d_bag_build_masked)."""
+    ctx = Context.get()
+    geoms = (N.BagGeom * len(lattices))(*[l.geom for l in lattices])
+    dgeoms = DeviceArray(C.sizeof(geoms))
+    N.check(N.lib().th2d_h2d(ctx.ptr, dgeoms.p, C.cast(geoms, C.c_void_p), C.sizeof(geoms)),
+            "th2d_h2d")
+    flat = np.concatenate([_i32(f) for f in fields])
+    foff = np.cumsum([0] + [len(f) for f in fields])[:-1].astype(np.int64)
+    dF, dFo = DeviceArray.from_host(flat), DeviceArray.from_host(foff)
+    db = DenseBatch(None, precision, orders=[l.V for l in lattices])
+    N.check(N.lib().th2d_bag_build_masked(ctx.ptr, dgeoms.p, dF.p, dFo.p, C.byref(db.batch),
+                                          db.A.p, precision), "th2d_bag_build_masked")
+    db._keep = (dgeoms, dF, dFo)
+    return db

@@ -501,9 +501,102 @@ template <typename T>
 int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
                 const c2<T> *const *rhs,
                 const c2<T> *const *K, const double *stop_rr, const int *max_iter, c2<T> **xout,
-                InnerResult *res)
+                InnerResult *res, int force_shape = -1);
+
+// Key of a batch for the autotuner: the distinct lattice shapes and the
+// order of magnitude of the site count (a restart pass over a subset of
+// the chains reuses the choice made for the whole batch).
+unsigned long long tune_key(int nch, const Geo *G)
 {
-  const int shape = choose_shape(ctx, nch, G);
+  std::vector<std::pair<int, int>> d;
+  long sites = 0;
+  for (int c = 0; c < nch; c++) {
+    d.emplace_back(G[c].nt, G[c].nx);
+    sites += 2L * G[c].vh;
+  }
+  std::sort(d.begin(), d.end());
+  d.erase(std::unique(d.begin(), d.end()), d.end());
+  unsigned long long h = 1469598103934665603ull;
+  auto mix = [&](unsigned long long v) { h = (h ^ v) * 1099511628211ull; };
+  for (auto &q : d) mix(((unsigned long long)q.first << 32) | (unsigned)q.second);
+  int lg = 0;
+  while ((1L << lg) < sites) lg++;
+  mix(lg);
+  return h;
+}
+
+// Measuring autotuner: runs `iters` solver iterations of the actual batch
+// with every compiled tile shape that fits, times them with HIP events and
+// returns the fastest (the solver state is rebuilt by the real solve).
+template <typename T>
+int autotune_shape(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T> *const *rhs,
+                   const c2<T> *const *K, int iters)
+{
+  int min_nt = 1 << 30, min_nx = 1 << 30;
+  for (int k = 0; k < nch; k++) {
+    min_nt = std::min(min_nt, G[k].nt);
+    min_nx = std::min(min_nx, G[k].nx);
+  }
+  std::vector<double> stop(nch, 0.0);
+  std::vector<int> mi(nch, iters);
+  std::vector<c2<T> *> xo(nch);
+  std::vector<InnerResult> ir(nch);
+  const int saved_timing = ctx->ktiming, saved_chunk = ctx->cg_chunk;
+  ctx->ktiming = 0;
+  ctx->cg_chunk = iters + 1;
+  int best = -1;
+  float best_ms = 1e30f;
+  int rc = TH2D_OK;
+  for (int s = 0; s < kNumShapes && rc == TH2D_OK; s++) {
+    if (kShapes[s].tt > min_nt || kShapes[s].tx > min_nx) continue;
+    float ms = 0.f;
+    for (int rep = 0; rep < 2 && rc == TH2D_OK; rep++) {  // the first run warms up
+      if (hipEventRecord(ctx->a0, ctx->stream) != hipSuccess) {
+        rc = TH2D_EHIP;
+        break;
+      }
+      rc = inner_solve<T>(ctx, buf, 7, nch, G, rhs, K, stop.data(), mi.data(), xo.data(),
+                          ir.data(), s);
+      if (rc != TH2D_OK) break;
+      if (hipEventRecord(ctx->a1, ctx->stream) != hipSuccess ||
+          hipEventSynchronize(ctx->a1) != hipSuccess ||
+          hipEventElapsedTime(&ms, ctx->a0, ctx->a1) != hipSuccess)
+        rc = TH2D_EHIP;
+    }
+    if (rc == TH2D_OK && ms < best_ms) {
+      best_ms = ms;
+      best = s;
+    }
+  }
+  ctx->ktiming = saved_timing;
+  ctx->cg_chunk = saved_chunk;
+  if (rc != TH2D_OK) return rc;
+  return best >= 0 ? best : TH2D_EARG;
+}
+
+template <typename T>
+int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
+                const c2<T> *const *rhs,
+                const c2<T> *const *K, const double *stop_rr, const int *max_iter, c2<T> **xout,
+                InnerResult *res, int force_shape)
+{
+  int shape = force_shape;
+  if (shape < 0 && ctx->cg_autotune && !(ctx->tile_t > 0 && ctx->tile_x > 0)) {
+    auto &cache = ctx->cg_tuned[sizeof(T) == 8 ? 1 : 0];
+    const unsigned long long key = tune_key(nch, G);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      shape = it->second;
+    } else {
+      int s = autotune_shape<T>(ctx, buf, nch, G, rhs, K, 12);
+      if (s < 0) return s;
+      cache[key] = s;
+      shape = s;
+    }
+    for (int c = 0; c < nch; c++)  // a shape larger than a lattice of this subset
+      if (kShapes[shape].tt > G[c].nt || kShapes[shape].tx > G[c].nx) shape = -1;
+  }
+  if (shape < 0) shape = choose_shape(ctx, nch, G);
   if (shape < 0) return TH2D_EARG;
   const ShapeEntry tc = kShapes[shape];
   const int nthreads = tc.nt;
@@ -665,6 +758,13 @@ extern "C" int th2d_set_cg_tile(th2d_ctx *ctx, int tt, int tx)
   if ((tt || tx) && shape_index(tt, tx) < 0) return TH2D_EARG;
   ctx->tile_t = tt;
   ctx->tile_x = tx;
+  return TH2D_OK;
+}
+
+extern "C" int th2d_set_cg_autotune(th2d_ctx *ctx, int on)
+{
+  if (!ctx) return TH2D_EARG;
+  ctx->cg_autotune = on ? 1 : 0;
   return TH2D_OK;
 }
 

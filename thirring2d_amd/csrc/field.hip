@@ -55,6 +55,8 @@ extern "C" int th2d_ctx_create(th2d_ctx **out, void *stream)
   TH2D_HIP(hipEventCreate(&c->t1));
   TH2D_HIP(hipEventCreate(&c->k0));
   TH2D_HIP(hipEventCreate(&c->k1));
+  TH2D_HIP(hipEventCreate(&c->a0));
+  TH2D_HIP(hipEventCreate(&c->a1));
   *out = c;
   return TH2D_OK;
 }
@@ -76,6 +78,8 @@ extern "C" int th2d_ctx_destroy(th2d_ctx *c)
   (void)hipEventDestroy(c->t1);
   (void)hipEventDestroy(c->k0);
   (void)hipEventDestroy(c->k1);
+  (void)hipEventDestroy(c->a0);
+  (void)hipEventDestroy(c->a1);
   delete c;
   return TH2D_OK;
 }

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 
 #include "../../include/th2d.h"
 
@@ -43,6 +44,7 @@ struct th2d_ctx {
   // per-kernel timing of the solver kernel
   int ktiming = 0;
   hipEvent_t k0 = nullptr, k1 = nullptr;
+  hipEvent_t a0 = nullptr, a1 = nullptr;  // autotuner
   double ktime_ms = 0.0;
   long klaunches = 0;
   double kbytes = 0.0;
@@ -55,6 +57,8 @@ struct th2d_ctx {
   long kiters = 0;
   int cg_pass_iters[2][8] = {};   // iterations per restart pass (FP32, FP64)
   int cg_last_tile[2] = {0, 0};   // tile shape of the last inner solve
+  int cg_autotune = 0;            // measure tile shapes per batch signature
+  std::map<unsigned long long, int> cg_tuned[2];  // signature -> shape (FP32, FP64)
 };
 
 int th2d_buf_reserve(th2d_buf &b, size_t bytes);
