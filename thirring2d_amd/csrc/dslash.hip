@@ -10,7 +10,7 @@
 // Two variants: a direct one (one thread per eo site; the t and x
 // neighbours of a wavefront are contiguous runs of the other parity) and an
 // LDS-tiled one that stages a (rows+2) x (cols+2) block of the input once
-// and serves all four neighbour reads from LDS.
+// and serves all four neighbour reads from LDS.  V = 2 G.vh < 2^31.
 #include "th2d_internal.h"
 
 template <typename T> struct HopCoef {
@@ -39,26 +39,45 @@ __device__ __forceinline__ HopCoef<T> hop_coefs(const Geo &G, c2<T> kt, c2<T> kx
 }
 
 // ------------------------------------------------------------- direct ---
-template <typename T, int DAG>
-__global__ void __launch_bounds__(256) k_dslash_direct(Geo G, c2<T> *__restrict__ out,
-                                                       const c2<T> *__restrict__ in,
-                                                       const c2<T> *__restrict__ K, int nrhs)
+// One thread per eo site; a workgroup covers NT consecutive sites of one
+// parity row.  Rows are ordered (t, parity) so that both parities of a t
+// run back to back: an output row of parity p reads the rows t-1, t, t+1
+// and the neighbour links of parity 1-p, which the other parity's rows of
+// the same t read (or wrote) a moment earlier, so they are still in L2.  A
+// parity-major order fetches the input and the links from HBM twice on
+// lattices larger than the MALL.  Workgroups are dispatched round-robin
+// over the 8 XCDs: with a multiple of 8 workgroups per row each XCD walks
+// a column strip of all rows; otherwise each XCD gets a contiguous range
+// of rows (xcd_block).
+__device__ __forceinline__ int xcd_block(int v, int n)
+{
+  const int q = n >> 3, rem = n & 7, x = v & 7, k = v >> 3;
+  return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + k;
+}
+
+template <typename T, int DAG, int NT>
+__global__ void __launch_bounds__(NT) k_dslash_direct(Geo G, c2<T> *__restrict__ out,
+                                                      const c2<T> *__restrict__ in,
+                                                      const c2<T> *__restrict__ K, int nrhs,
+                                                      int bpr)
 {
   using V2 = c2<T>;
   const int V = 2 * G.vh;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= V) return;
-  int p = i >= G.vh;
-  int h = i - p * G.vh;
-  int t = h / G.nxh, j = h - t * G.nxh;
-  int x = 2 * j + ((t + p) & 1);
-  int q = (1 - p) * G.vh;
-  int tu = t + 1 == G.nt ? 0 : t + 1, td = t == 0 ? G.nt - 1 : t - 1;
-  int xu = x + 1 == G.nx ? 0 : x + 1, xd = x == 0 ? G.nx - 1 : x - 1;
-  int itu = q + tu * G.nxh + j, itd = q + td * G.nxh + j;
-  int ixu = q + t * G.nxh + (xu >> 1), ixd = q + t * G.nxh + (xd >> 1);
+  // with a multiple of 8 workgroups per row the dispatch order already
+  // keeps a column strip of rows on one XCD
+  const int lb = (bpr & 7) ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int row = lb / bpr, bx = lb - row * bpr;
+  const int p = row & 1, t = row >> 1;
+  const int j = bx * NT + threadIdx.x;
+  if (j >= G.nxh) return;
+  const int i = p * G.vh + t * G.nxh + j, q = (1 - p) * G.vh;
+  const int x = 2 * j + ((t + p) & 1);
+  const int tu = t + 1 == G.nt ? 0 : t + 1, td = t == 0 ? G.nt - 1 : t - 1;
+  const int xu = x + 1 == G.nx ? 0 : x + 1, xd = x == 0 ? G.nx - 1 : x - 1;
+  const int itu = q + tu * G.nxh + j, itd = q + td * G.nxh + j;
+  const int ixu = q + t * G.nxh + (xu >> 1), ixd = q + t * G.nxh + (xd >> 1);
   HopCoef<T> c = hop_coefs<T, DAG>(G, K[i], K[V + i], K[itd], K[V + ixd]);
-  T m = (T)G.m;
+  const T m = (T)G.m;
   for (int r = 0; r < nrhs; r++) {
     const V2 *src = in + (long)r * V;
     V2 v = cscale(src[i], m);
@@ -169,8 +188,21 @@ static void launch_t(th2d_ctx *c, const Geo &G, void *out, const void *in, const
     hipLaunchKernelGGL((k_dslash_lds<T, DAG, R, CC>), dim3(tiles), dim3(R * CC), 0, c->stream, G,
                        (V2 *)out, (const V2 *)in, (const V2 *)K, nrhs);
   } else {
-    hipLaunchKernelGGL((k_dslash_direct<T, DAG>), dim3((V + 255) / 256), dim3(256), 0, c->stream,
-                       G, (V2 *)out, (const V2 *)in, (const V2 *)K, nrhs);
+    // FP64: 256 threads per workgroup, FP32: 128 (measured at 2048^2 and
+    // 4096^2); narrower rows take the smallest multiple of 64 that covers them
+    const int nt_max = sizeof(T) == 8 ? 256 : 128;
+    const int nt = G.nxh >= nt_max ? nt_max : (G.nxh > 64 ? 128 : 64);
+    const int bpr = (G.nxh + nt - 1) / nt;
+    const dim3 grid((unsigned)((long)bpr * 2 * G.nt));
+    if (nt == 256)
+      hipLaunchKernelGGL((k_dslash_direct<T, DAG, 256>), grid, dim3(256), 0, c->stream, G,
+                         (V2 *)out, (const V2 *)in, (const V2 *)K, nrhs, bpr);
+    else if (nt == 128)
+      hipLaunchKernelGGL((k_dslash_direct<T, DAG, 128>), grid, dim3(128), 0, c->stream, G,
+                         (V2 *)out, (const V2 *)in, (const V2 *)K, nrhs, bpr);
+    else
+      hipLaunchKernelGGL((k_dslash_direct<T, DAG, 64>), grid, dim3(64), 0, c->stream, G,
+                         (V2 *)out, (const V2 *)in, (const V2 *)K, nrhs, bpr);
   }
 }
 
