@@ -726,29 +726,8 @@ int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
   return TH2D_OK;
 }
 
-__global__ void k_scale_f64_to_f32(float2 *d, const double2 *s, long n, double sc)
-{
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) d[i] = make_float2((float)(s[i].x * sc), (float)(s[i].y * sc));
-}
 
-__global__ void k_axpy_f32_into_f64(double2 *y, const float2 *x, long n, double a)
-{
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    double2 v = y[i];
-    y[i] = make_double2(v.x + a * (double)x[i].x, v.y + a * (double)x[i].y);
-  }
-}
 
-__global__ void k_axpy_f64(double2 *y, const double2 *x, long n, double a)
-{
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    double2 v = y[i];
-    y[i] = make_double2(v.x + a * x[i].x, v.y + a * x[i].y);
-  }
-}
 
 }  // namespace
 
@@ -824,10 +803,37 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
   std::vector<int> left(nch, p->max_iter), active(nch, 1), status(nch, TH2D_OK);
   const bool abs_mode = p->tol_rel <= 0;
   const double delta = p->delta > 0 ? p->delta : 1e-4;
-  for (int c = 0; c < nch; c++) {
-    memset(&inf[c], 0, sizeof(th2d_cg_info));
-    TH2D_TRY(th2d::reduce_dot(ctx, 2 * V[c], b[c], b[c], dsc + c));
-    TH2D_TRY(th2d_memset(ctx, x[c], 0, sizeof(double) * 2 * V[c]));
+  // per-chain descriptor of a batched site operation (batch.hip)
+  auto op_of = [&](int c) {
+    BOp o;
+    memset(&o, 0, sizeof(o));
+    o.G = G[c];
+    o.bc = Bc{geoms[c].anti_t, geoms[c].anti_x};
+    return o;
+  };
+  {
+    // |b|^2 and x = 0 for every chain: two batched launches
+    std::vector<BOp> t0(nch), t1(nch);
+    for (int c = 0; c < nch; c++) {
+      memset(&inf[c], 0, sizeof(th2d_cg_info));
+      t0[c] = op_of(c);
+      t0[c].d0 = (double *)b[c];
+      t0[c].d1 = (double *)b[c];
+      t0[c].s1 = 1.0;
+      t0[c].out = dsc + c;
+      t1[c] = op_of(c);  // x = b - b
+      t1[c].d0 = x[c];
+      t1[c].d1 = (double *)b[c];
+      t1[c].d2 = (double *)b[c];
+    }
+    const long nblk = th2d::bop_finish(t0);
+    th2d::bop_finish(t1);
+    std::vector<BOp> all(t0);
+    all.insert(all.end(), t1.begin(), t1.end());
+    TH2D_TRY(th2d::bop_upload(ctx, ctx->btab[0], all));
+    const BOp *dt = (const BOp *)ctx->btab[0].ptr;
+    TH2D_TRY(th2d::bop_launch(ctx, th2d::BOP_DOT, dt, nch, nblk));
+    TH2D_TRY(th2d::bop_launch(ctx, th2d::BOP_SUB, dt + nch, nch, nblk));
   }
   TH2D_TRY(th2d::fetch_doubles(ctx, bb.data(), dsc, nch));
   for (int c = 0; c < nch; c++) {
@@ -849,23 +855,36 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
     std::vector<double> stop(na);
     std::vector<int> mi(na);
     std::vector<InnerResult> ir(na);
+    // right-hand sides of the inner solves: r / |r| (FP32 or FP64)
+    std::vector<BOp> tin(na);
     for (int k = 0; k < na; k++) {
       int c = idx[k];
       g2[k] = G[c];
       mi[k] = left[c];
       const double *rhs = outer == 0 ? b[c] : R + 2 * voff[c];
-      double nr = sqrt(rr[c]);
+      const double nr = sqrt(rr[c]);
+      tin[k] = op_of(c);
+      tin[k].d1 = (double *)rhs;
+      tin[k].s0 = 1.0 / nr;
       if (mixed) {
-        hipLaunchKernelGGL(k_scale_f64_to_f32, dim3((unsigned)((V[c] + 255) / 256)), dim3(256), 0,
-                           ctx->stream, R32 + voff[c], (const double2 *)rhs, V[c], 1.0 / nr);
+        tin[k].f0 = (float *)(R32 + voff[c]);
         stop[k] = std::max(delta * delta, target[c] / rr[c]);
       } else if (outer == 0) {
         stop[k] = target[c];  // unscaled first pass: the reference criterion
       } else {
-        TH2D_TRY(th2d::axpy_like(ctx, 2, 2 * V[c], 1.0 / nr, TMP + 2 * voff[c], rhs, nullptr));
+        tin[k].d0 = TMP + 2 * voff[c];
         stop[k] = abs_mode ? target[c] / rr[c] : std::max(target[c] / rr[c], 1e-30);
       }
     }
+    const long nblk_in = th2d::bop_finish(tin);
+    if (mixed || outer > 0) {
+      TH2D_TRY(th2d::bop_upload(ctx, ctx->btab[0], tin));
+      TH2D_TRY(th2d::bop_launch(ctx, mixed ? th2d::BOP_TO_F32 : th2d::BOP_SCALE,
+                                (const BOp *)ctx->btab[0].ptr, na, nblk_in));
+    }
+    // inner solve, then x += |r| y for every chain and the true residual
+    // r = b - M^dag M x in FP64: five batched launches
+    std::vector<BOp> tx(na), tm(na), tmd(na), tsub(na), tdot(na);
     if (mixed) {
       std::vector<const float2 *> rh(na), kk(na);
       std::vector<float2 *> xo(na);
@@ -873,12 +892,13 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
         rh[k] = R32 + voff[idx[k]];
         kk[k] = (const float2 *)K32[idx[k]];
       }
-      TH2D_TRY(inner_solve<float>(ctx, ctx->cg32, outer, na, g2.data(), rh.data(), kk.data(), stop.data(),
-                                  mi.data(), xo.data(), ir.data()));
+      TH2D_TRY(inner_solve<float>(ctx, ctx->cg32, outer, na, g2.data(), rh.data(), kk.data(),
+                                  stop.data(), mi.data(), xo.data(), ir.data()));
       for (int k = 0; k < na; k++) {
-        int c = idx[k];
-        hipLaunchKernelGGL(k_axpy_f32_into_f64, dim3((unsigned)((V[c] + 255) / 256)), dim3(256), 0,
-                           ctx->stream, (double2 *)x[c], xo[k], V[c], sqrt(rr[c]));
+        tx[k] = op_of(idx[k]);
+        tx[k].d0 = x[idx[k]];
+        tx[k].f0 = (float *)xo[k];
+        tx[k].s0 = sqrt(rr[idx[k]]);
       }
     } else {
       std::vector<const double2 *> rh(na), kk(na);
@@ -891,22 +911,44 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
       TH2D_TRY(inner_solve<double>(ctx, ctx->cg64, outer, na, g2.data(), rh.data(), kk.data(),
                                    stop.data(), mi.data(), xo.data(), ir.data()));
       for (int k = 0; k < na; k++) {
-        int c = idx[k];
-        hipLaunchKernelGGL(k_axpy_f64, dim3((unsigned)((V[c] + 255) / 256)), dim3(256), 0,
-                           ctx->stream, (double2 *)x[c], xo[k], V[c],
-                           outer > 0 ? sqrt(rr[c]) : 1.0);
+        tx[k] = op_of(idx[k]);
+        tx[k].d0 = x[idx[k]];
+        tx[k].d1 = (double *)xo[k];
+        tx[k].s0 = outer > 0 ? sqrt(rr[idx[k]]) : 1.0;
       }
     }
-    TH2D_HIP(hipGetLastError());
-    // true residuals r = b - A x in FP64
     for (int k = 0; k < na; k++) {
       int c = idx[k];
       double *r = R + 2 * voff[c], *tmp = TMP + 2 * voff[c];
-      TH2D_TRY(th2d::dslash_launch(ctx, G[c], tmp, x[c], K[c], 0, TH2D_PREC_F64, 1));
-      TH2D_TRY(th2d::dslash_launch(ctx, G[c], r, tmp, K[c], 1, TH2D_PREC_F64, 1));
-      TH2D_TRY(th2d::axpy_like(ctx, 1, 2 * V[c], 0.0, r, b[c], r));
-      TH2D_TRY(th2d::reduce_dot(ctx, 2 * V[c], r, r, dsc + k));
+      tm[k] = op_of(c);  // tmp = M x
+      tm[k].d0 = tmp;
+      tm[k].d1 = x[c];
+      tm[k].d2 = (double *)K[c];
+      tmd[k] = tm[k];  // r = M^dag tmp
+      tmd[k].d0 = r;
+      tmd[k].d1 = tmp;
+      tmd[k].s1 = 1.0;
+      tsub[k] = op_of(c);  // r = b - r
+      tsub[k].d0 = r;
+      tsub[k].d1 = (double *)b[c];
+      tsub[k].d2 = r;
+      tdot[k] = op_of(c);  // |r|^2
+      tdot[k].d0 = r;
+      tdot[k].d1 = r;
+      tdot[k].s1 = 1.0;
+      tdot[k].out = dsc + k;
     }
+    const long nblk = th2d::bop_finish(tx);
+    for (auto *t : {&tm, &tmd, &tsub, &tdot}) th2d::bop_finish(*t);
+    std::vector<BOp> all(tx);
+    for (auto *t : {&tm, &tmd, &tsub, &tdot}) all.insert(all.end(), t->begin(), t->end());
+    TH2D_TRY(th2d::bop_upload(ctx, ctx->btab[1], all));
+    const BOp *dt = (const BOp *)ctx->btab[1].ptr;
+    TH2D_TRY(th2d::bop_launch(ctx, mixed ? th2d::BOP_ADD_F32 : th2d::BOP_AXPY, dt, na, nblk));
+    TH2D_TRY(th2d::bop_launch(ctx, th2d::BOP_DSLASH, dt + na, na, nblk));
+    TH2D_TRY(th2d::bop_launch(ctx, th2d::BOP_DSLASH, dt + 2 * na, na, nblk));
+    TH2D_TRY(th2d::bop_launch(ctx, th2d::BOP_SUB, dt + 3 * na, na, nblk));
+    TH2D_TRY(th2d::bop_launch(ctx, th2d::BOP_DOT, dt + 4 * na, na, nblk));
     std::vector<double> rn(na);
     TH2D_TRY(th2d::fetch_doubles(ctx, rn.data(), dsc, na));
     for (int k = 0; k < na; k++) {

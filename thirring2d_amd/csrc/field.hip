@@ -74,6 +74,8 @@ extern "C" int th2d_ctx_destroy(th2d_ctx *c)
   buf_release(c->force);
   buf_release(c->bag);
   buf_release(c->misc);
+  buf_release(c->bpart);
+  for (auto &b : c->btab) buf_release(b);
   (void)hipEventDestroy(c->t0);
   (void)hipEventDestroy(c->t1);
   (void)hipEventDestroy(c->k0);
@@ -341,37 +343,7 @@ extern "C" int th2d_vec_f32_to_f64(th2d_ctx *c, double *d, const float *s, long 
 }
 
 // ======================================================= gauge field ====
-// K_mu(x) = 1/2 eta_mu(x) s_mu(x) e^{i A_mu(x)}: eta_t = (-1)^x, eta_x = 1
-// (hmc.c:914-921); s = -1 for the hop leaving the last row/column when
-// that direction is antiperiodic (hmc.c:143-180).
-struct Bc {
-  int anti_t, anti_x;
-};
-
-__device__ __forceinline__ void fold_one(const Geo &G, Bc bc, int i, const double *A, double *K,
-                                         float *K32)
-{
-  int V = 2 * G.vh;
-  int p = i >= G.vh;
-  int h = i - p * G.vh;
-  int t = h / G.nxh, j = h - t * G.nxh;
-  int x = 2 * j + ((t + p) & 1);
-  double ht = 0.5 * ((x & 1) ? -1.0 : 1.0);
-  if (bc.anti_t && t == G.nt - 1) ht = -ht;
-  double hx = 0.5;
-  if (bc.anti_x && x == G.nx - 1) hx = -hx;
-  double st, ct, sx, cx;
-  sincos(A[i], &st, &ct);
-  sincos(A[V + i], &sx, &cx);
-  double2 kt = make_double2(ht * ct, ht * st), kx = make_double2(hx * cx, hx * sx);
-  ((double2 *)K)[i] = kt;
-  ((double2 *)K)[V + i] = kx;
-  if (K32) {
-    ((float2 *)K32)[i] = make_float2((float)kt.x, (float)kt.y);
-    ((float2 *)K32)[V + i] = make_float2((float)kx.x, (float)kx.y);
-  }
-}
-
+// Folding of the links: fold_one (th2d_internal.h).
 __global__ void k_fold(Geo G, Bc bc, const double *A, double *K, float *K32)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -22,41 +22,10 @@ __global__ void __launch_bounds__(256) k_force(Geo G, int anti_t, int anti_x,
                                                double *__restrict__ fout)
 {
   const int V = 2 * G.vh;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= V) return;
-  int p = i >= G.vh;
-  int h = i - p * G.vh;
-  int t = h / G.nxh, j = h - t * G.nxh;
-  int x = 2 * j + ((t + p) & 1);
-  int q = (1 - p) * G.vh;
-  int tu = t + 1 == G.nt ? 0 : t + 1;
-  int xu = x + 1 == G.nx ? 0 : x + 1;
-  int itu = q + tu * G.nxh + j, ixu = q + t * G.nxh + (xu >> 1);
-  double2 c0 = chi[i], m0 = mchi[i];
   double F[2];
-  for (int dir = 0; dir < 2; dir++) {
-    int in = dir == 0 ? itu : ixu;
-    double2 k = K[dir * V + i];
-    double2 c1 = chi[in], m1 = mchi[in];
-    double ef = dir == 0 ? G.ep : 1.0, eb = dir == 0 ? G.em : 1.0;
-    // z = e^{+} conj(m0) c1 - e^{-} conj(c0) m1
-    double2 a = cjmul(m0, c1), b = cjmul(c0, m1);
-    double2 z = make_double2(ef * a.x - eb * b.x, ef * a.y - eb * b.y);
-    // Re[i k z] = -Im(k z)
-    double im_kz = k.x * z.y + k.y * z.x;
-    double f = 2.0 * im_kz;
-    // gauge force: sin A = 2 eta s Im K
-    double eta = dir == 0 ? ((x & 1) ? -1.0 : 1.0) : 1.0;
-    int wraps = dir == 0 ? (anti_t && t == G.nt - 1) : (anti_x && x == G.nx - 1);
-    double sgn = wraps ? -eta : eta;
-    f += cg * 2.0 * sgn * k.y;
-    if (st && dir == 0 && ef != eb) {
-      double2 cst = cjmul(st[i], st[in]);
-      double im = k.x * cst.y + k.y * cst.x;
-      f += (ef - eb) * (-im);
-    }
-    F[dir] = f;
-  }
+  force_site(G, Bc{anti_t, anti_x}, K, chi, mchi, st, cg, i, F);
   if (fout) {
     fout[i] = F[0];
     fout[V + i] = F[1];

@@ -8,7 +8,8 @@
 //   S_pf1 = Re psi^dag (M^dag M)^{-1} psi, S_mom1, S_g1, S_md1
 //   dH = sum of differences; accept if exp(-dH) > u.
 // The chains of a batch advance in lockstep: every inversion is one batched
-// solve (cg.hip), the field updates and forces are per-chain launches.
+// solve (cg.hip), and every field update, force, operator application and
+// action sum is one launch over all chains (batch.hip).
 // Fields stay on the device for the whole trajectory; the host sees the
 // solver's iteration counts and the action sums.
 #include "th2d_internal.h"
@@ -18,18 +19,11 @@
 #include <vector>
 
 namespace th2d {
-int reduce_op(th2d_ctx *c, int op, long n, const double *a, const double *b, double *dout,
-              double scale);
 int gauss_fill(th2d_ctx *c, double *v, long n_complex, unsigned long long seed, uint32_t stream,
                uint32_t ctr, double scale);
-int force_launch(th2d_ctx *c, const Geo &G, int anti_t, int anti_x, double *mom, const double *K,
-                 const double *chi, const double *st, double eps, double gc, int nf, double *fout,
-                 double *mchi_scratch);
 }  // namespace th2d
 
 namespace {
-
-enum { RED_DOT = 0, RED_COSGAP = 2 };
 
 struct ChainBufs {
   double *nA, *K, *mom, *xi, *psi, *st, *chi, *tmp, *mchi, *stage;
@@ -135,21 +129,70 @@ extern "C" int th2d_hmc_trajectory_batched(th2d_ctx *c, int nch, const th2d_geom
   }
 
   const double h = h0.traj_len / h0.nsteps;
-  // ---- start of trajectory ----
+  // ---- descriptor tables of the batched site operations (batch.hip) ----
+  // one table per operation, all chains in each; uploaded once
+  enum { T_FOLD, T_PSI, T_DXI, T_DMOM0, T_COS0, T_TMP, T_DST, T_COPY, T_GS, T_MCHI, T_FORCE,
+         T_DPC, T_DMOM1, T_COS1, T_DST1, T_N };
+  std::vector<std::vector<BOp>> T(T_N, std::vector<BOp>(nch));
   for (int k = 0; k < nch; k++) {
     ChainBufs &b = B[k];
-    const double gnf = hps[k].nf / hps[k].g;
+    BOp o;
+    memset(&o, 0, sizeof(o));
+    o.G = G[k];
+    o.bc = Bc{geos[k].anti_t, geos[k].anti_x};
     double *a = dacts + 8 * k;
-    TH2D_TRY(th2d_fold_links(c, &geos[k], b.K, mixed ? b.K32 : nullptr, A[k]));
-    TH2D_TRY(th2d::dslash_launch(c, G[k], b.psi, b.xi, b.K, 1, TH2D_PREC_F64, 1));
-    TH2D_TRY(th2d::reduce_op(c, RED_DOT, 2 * V[k], b.xi, b.xi, a + 1, 1.0));
-    TH2D_TRY(th2d::reduce_op(c, RED_DOT, 2 * V[k], b.mom, b.mom, a + 3, 1.0));
-    TH2D_TRY(th2d::reduce_op(c, RED_COSGAP, 2 * V[k], A[k], nullptr, a + 0, gnf));
-    TH2D_TRY(th2d::dslash_launch(c, G[k], b.tmp, b.st, b.K, 1, TH2D_PREC_F64, 1));
-    TH2D_TRY(th2d::reduce_op(c, RED_DOT, 2 * V[k], b.st, b.tmp, a + 2, 1.0));
-    TH2D_HIP(hipMemcpyAsync(b.nA, A[k], sizeof(double) * 2 * V[k], hipMemcpyDeviceToDevice,
-                            c->stream));
+    const double gnf = hps[k].nf / hps[k].g;
+    float *k32 = mixed ? b.K32 : nullptr;
+    auto set = [&](int t, double *d0, double *d1, double *d2, double *d3, double *d4, float *f0,
+                   double s0, double s1, double *out) {
+      BOp q = o;
+      q.d0 = d0;
+      q.d1 = d1;
+      q.d2 = d2;
+      q.d3 = d3;
+      q.d4 = d4;
+      q.f0 = f0;
+      q.s0 = s0;
+      q.s1 = s1;
+      q.out = out;
+      T[t][k] = q;
+    };
+    set(T_FOLD, A[k], nullptr, b.K, nullptr, nullptr, k32, 0, 0, nullptr);
+    set(T_PSI, b.psi, b.xi, b.K, nullptr, nullptr, nullptr, 0, 1.0, nullptr);  // M^dag xi
+    set(T_DXI, b.xi, b.xi, nullptr, nullptr, nullptr, nullptr, 0, 1.0, a + 1);
+    set(T_DMOM0, b.mom, b.mom, nullptr, nullptr, nullptr, nullptr, 0, 1.0, a + 3);
+    set(T_COS0, A[k], nullptr, nullptr, nullptr, nullptr, nullptr, 0, gnf, a + 0);
+    set(T_TMP, b.tmp, b.st, b.K, nullptr, nullptr, nullptr, 0, 1.0, nullptr);  // M^dag st
+    set(T_DST, b.st, b.tmp, nullptr, nullptr, nullptr, nullptr, 0, 1.0, a + 2);
+    set(T_COPY, b.nA, A[k], nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr);
+    set(T_GS, b.nA, b.mom, b.K, nullptr, nullptr, k32, 0.5 * h, 0, nullptr);
+    set(T_MCHI, b.mchi, b.chi, b.K, nullptr, nullptr, nullptr, 0, 0.0, nullptr);  // M chi
+    set(T_FORCE, b.mom, b.chi, b.K, b.mchi, geos[k].mu != 0.0 ? b.st : nullptr, nullptr, h, gnf,
+        nullptr);
+    set(T_DPC, b.psi, b.chi, nullptr, nullptr, nullptr, nullptr, 0, 1.0, a + 5);
+    set(T_DMOM1, b.mom, b.mom, nullptr, nullptr, nullptr, nullptr, 0, 1.0, a + 7);
+    set(T_COS1, b.nA, nullptr, nullptr, nullptr, nullptr, nullptr, 0, gnf, a + 4);
+    set(T_DST1, b.st, b.tmp, nullptr, nullptr, nullptr, nullptr, 0, 1.0, a + 6);
   }
+  long nblk = 0;
+  std::vector<BOp> all;
+  for (auto &t : T) {
+    nblk = th2d::bop_finish(t);
+    all.insert(all.end(), t.begin(), t.end());
+  }
+  TH2D_TRY(th2d::bop_upload(c, c->btab[2], all));
+  const BOp *dt = (const BOp *)c->btab[2].ptr;
+  auto run = [&](int t, int op) { return th2d::bop_launch(c, op, dt + (size_t)t * nch, nch, nblk); };
+
+  // ---- start of trajectory ----
+  TH2D_TRY(run(T_FOLD, th2d::BOP_FOLD));
+  TH2D_TRY(run(T_PSI, th2d::BOP_DSLASH));
+  TH2D_TRY(run(T_DXI, th2d::BOP_DOT));
+  TH2D_TRY(run(T_DMOM0, th2d::BOP_DOT));
+  TH2D_TRY(run(T_COS0, th2d::BOP_COSGAP));
+  TH2D_TRY(run(T_TMP, th2d::BOP_DSLASH));
+  TH2D_TRY(run(T_DST, th2d::BOP_DOT));
+  TH2D_TRY(run(T_COPY, th2d::BOP_COPY));
 
   std::vector<double *> xs(nch);
   std::vector<const double *> bs(nch), ks(nch);
@@ -179,31 +222,20 @@ extern "C" int th2d_hmc_trajectory_batched(th2d_ctx *c, int nch, const th2d_geom
 
   // ---- leapfrog (hmc.c:708-716) ----
   for (int s = 0; s < h0.nsteps; s++) {
-    for (int k = 0; k < nch; k++)
-      TH2D_TRY(th2d_gauge_step(c, &geos[k], B[k].nA, B[k].mom, 0.5 * h, B[k].K,
-                               mixed ? B[k].K32 : nullptr));
+    TH2D_TRY(run(T_GS, th2d::BOP_GAUGE_STEP));
     TH2D_TRY(solve());
-    for (int k = 0; k < nch; k++)
-      TH2D_TRY(th2d::force_launch(c, G[k], geos[k].anti_t, geos[k].anti_x, B[k].mom, B[k].K,
-                                  B[k].chi, geos[k].mu != 0.0 ? B[k].st : nullptr, h, hps[k].g,
-                                  hps[k].nf, nullptr, B[k].mchi));
-    for (int k = 0; k < nch; k++)
-      TH2D_TRY(th2d_gauge_step(c, &geos[k], B[k].nA, B[k].mom, 0.5 * h, B[k].K,
-                               mixed ? B[k].K32 : nullptr));
+    TH2D_TRY(run(T_MCHI, th2d::BOP_DSLASH));
+    TH2D_TRY(run(T_FORCE, th2d::BOP_FORCE));
+    TH2D_TRY(run(T_GS, th2d::BOP_GAUGE_STEP));
   }
 
   // ---- end of trajectory ----
   TH2D_TRY(solve());
-  for (int k = 0; k < nch; k++) {
-    ChainBufs &b = B[k];
-    double *a = dacts + 8 * k;
-    TH2D_TRY(th2d::reduce_op(c, RED_DOT, 2 * V[k], b.psi, b.chi, a + 5, 1.0));
-    TH2D_TRY(th2d::reduce_op(c, RED_DOT, 2 * V[k], b.mom, b.mom, a + 7, 1.0));
-    TH2D_TRY(th2d::reduce_op(c, RED_COSGAP, 2 * V[k], b.nA, nullptr, a + 4,
-                             hps[k].nf / hps[k].g));
-    TH2D_TRY(th2d::dslash_launch(c, G[k], b.tmp, b.st, b.K, 1, TH2D_PREC_F64, 1));
-    TH2D_TRY(th2d::reduce_op(c, RED_DOT, 2 * V[k], b.st, b.tmp, a + 6, 1.0));
-  }
+  TH2D_TRY(run(T_DPC, th2d::BOP_DOT));
+  TH2D_TRY(run(T_DMOM1, th2d::BOP_DOT));
+  TH2D_TRY(run(T_COS1, th2d::BOP_COSGAP));
+  TH2D_TRY(run(T_TMP, th2d::BOP_DSLASH));
+  TH2D_TRY(run(T_DST1, th2d::BOP_DOT));
   std::vector<double> acts(8 * (size_t)nch);
   TH2D_TRY(th2d::fetch_doubles(c, acts.data(), dacts, 8 * nch));
   for (int k = 0; k < nch; k++) {
@@ -212,9 +244,23 @@ extern "C" int th2d_hmc_trajectory_batched(th2d_ctx *c, int nch, const th2d_geom
     R[k].dS = dS;
     memcpy(R[k].acts, a, sizeof(double) * 8);
     R[k].accepted = exp(-dS) > u_accept[k];
-    if (R[k].accepted)
-      TH2D_HIP(hipMemcpyAsync(A[k], B[k].nA, sizeof(double) * 2 * V[k], hipMemcpyDeviceToDevice,
-                              c->stream));
+  }
+  // accepted chains take the evolved field (one batched copy)
+  std::vector<BOp> tacc;
+  for (int k = 0; k < nch; k++)
+    if (R[k].accepted) {
+      BOp q;
+      memset(&q, 0, sizeof(q));
+      q.G = G[k];
+      q.d0 = A[k];
+      q.d1 = B[k].nA;
+      tacc.push_back(q);
+    }
+  if (!tacc.empty()) {
+    const long nb = th2d::bop_finish(tacc);
+    TH2D_TRY(th2d::bop_upload(c, c->btab[3], tacc));
+    TH2D_TRY(th2d::bop_launch(c, th2d::BOP_COPY, (const BOp *)c->btab[3].ptr, (int)tacc.size(),
+                              nb));
   }
   TH2D_HIP(hipStreamSynchronize(c->stream));
   int worst = TH2D_OK;

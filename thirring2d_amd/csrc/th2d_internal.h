@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <vector>
 
 #include "../../include/th2d.h"
 
@@ -50,6 +51,7 @@ struct th2d_ctx {
   double kbytes = 0.0;
   // scratch
   th2d_buf cg64, cg32, cgmix, red, redh, hmc, force, bag, misc;
+  th2d_buf bpart, btab[4];  // batched site operations: partials, descriptor tables
   // tile configuration of the persistent solver (0 = autotuned default)
   int tile_t = 0, tile_x = 0;
   int cg_max_blocks = 0;  // 0 = one workgroup per tile
@@ -174,6 +176,135 @@ __device__ __forceinline__ double2 block_sum2(double a, double b, double *sh)
   return s;
 }
 
+// ------------------------------------------------------- per-site code --
+// Site of eo index i: (t, x) and the eo indices of its t- and x-neighbours.
+struct EoSite {
+  int p, t, x, j;
+  int itu, itd, ixu, ixd;  // neighbours (other parity)
+};
+
+__device__ __forceinline__ EoSite eo_site(const Geo &G, int i)
+{
+  EoSite s;
+  s.p = i >= G.vh;
+  const int h = i - s.p * G.vh;
+  s.t = h / G.nxh;
+  s.j = h - s.t * G.nxh;
+  s.x = 2 * s.j + ((s.t + s.p) & 1);
+  const int q = (1 - s.p) * G.vh;
+  const int tu = s.t + 1 == G.nt ? 0 : s.t + 1, td = s.t == 0 ? G.nt - 1 : s.t - 1;
+  const int xu = s.x + 1 == G.nx ? 0 : s.x + 1, xd = s.x == 0 ? G.nx - 1 : s.x - 1;
+  s.itu = q + tu * G.nxh + s.j;
+  s.itd = q + td * G.nxh + s.j;
+  s.ixu = q + s.t * G.nxh + (xu >> 1);
+  s.ixd = q + s.t * G.nxh + (xd >> 1);
+  return s;
+}
+
+// (M psi)(i) (dagger 0) or (M^dag psi)(i) (dagger 1) in FP64 with folded K.
+__device__ __forceinline__ double2 dslash_site(const Geo &G, const double2 *__restrict__ in,
+                                               const double2 *__restrict__ K, int i, int dagger)
+{
+  const int V = 2 * G.vh;
+  const EoSite s = eo_site(G, i);
+  const double2 kt = K[i], kx = K[V + i], ktd = K[s.itd], kxd = K[V + s.ixd];
+  // M:     + e^{mu} K_t psi(x+t) - e^{-mu} K_t(x-t)* psi(x-t) + K_x psi(x+x) - K_x(x-x)* psi(x-x)
+  // M^dag: - e^{-mu} K_t psi(x+t) + e^{mu} K_t(x-t)* psi(x-t) - K_x psi(x+x) + K_x(x-x)* psi(x-x)
+  const double fu = dagger ? -G.em : G.ep, fd = dagger ? -G.ep : G.em, sx = dagger ? -1.0 : 1.0;
+  double2 v = cscale(in[i], G.m);
+  v = cadd(v, cscale(cmul(kt, in[s.itu]), fu));
+  v = csub(v, cscale(cjmul(ktd, in[s.itd]), fd));
+  v = cadd(v, cscale(cmul(kx, in[s.ixu]), sx));
+  v = csub(v, cscale(cjmul(kxd, in[s.ixd]), sx));
+  return v;
+}
+
+// K_mu(x) = 1/2 eta_mu(x) s_mu(x) e^{i A_mu(x)}: eta_t = (-1)^x, eta_x = 1
+// (hmc.c:914-921); s = -1 for the hop leaving the last row/column when
+// that direction is antiperiodic (hmc.c:143-180).
+struct Bc {
+  int anti_t, anti_x;
+};
+
+__device__ __forceinline__ void fold_one(const Geo &G, Bc bc, int i, const double *A, double *K,
+                                         float *K32)
+{
+  const int V = 2 * G.vh;
+  const EoSite s = eo_site(G, i);
+  double ht = 0.5 * ((s.x & 1) ? -1.0 : 1.0);
+  if (bc.anti_t && s.t == G.nt - 1) ht = -ht;
+  double hx = 0.5;
+  if (bc.anti_x && s.x == G.nx - 1) hx = -hx;
+  double st, ct, sx, cx;
+  sincos(A[i], &st, &ct);
+  sincos(A[V + i], &sx, &cx);
+  double2 kt = make_double2(ht * ct, ht * st), kx = make_double2(hx * cx, hx * sx);
+  ((double2 *)K)[i] = kt;
+  ((double2 *)K)[V + i] = kx;
+  if (K32) {
+    ((float2 *)K32)[i] = make_float2((float)kt.x, (float)kt.y);
+    ((float2 *)K32)[V + i] = make_float2((float)kx.x, (float)kx.y);
+  }
+}
+
+// dS/dA of both links of site i (momentum_step, hmc.c:504-665):
+//   Nf/g sin A - 2 Re[i K (e^{+} conj(Mchi(x)) chi(x+mu) - e^{-} conj(chi(x)) Mchi(x+mu))]
+//   + (e^{mu} - e^{-mu}) Re[i K conj(st(x)) st(x+mu)]   (t only)
+// with sin A = 2 eta s Im K.  cg = Nf/g; st may be NULL.
+__device__ __forceinline__ void force_site(const Geo &G, Bc bc, const double2 *__restrict__ K,
+                                           const double2 *__restrict__ chi,
+                                           const double2 *__restrict__ mchi,
+                                           const double2 *__restrict__ st, double cg, int i,
+                                           double F[2])
+{
+  const int V = 2 * G.vh;
+  const EoSite s = eo_site(G, i);
+  const double2 c0 = chi[i], m0 = mchi[i];
+  for (int dir = 0; dir < 2; dir++) {
+    const int in = dir == 0 ? s.itu : s.ixu;
+    const double2 k = K[dir * V + i];
+    const double2 c1 = chi[in], m1 = mchi[in];
+    const double ef = dir == 0 ? G.ep : 1.0, eb = dir == 0 ? G.em : 1.0;
+    const double2 a = cjmul(m0, c1), b = cjmul(c0, m1);
+    const double2 z = make_double2(ef * a.x - eb * b.x, ef * a.y - eb * b.y);
+    double f = 2.0 * (k.x * z.y + k.y * z.x);  // -2 Re[i k z] = 2 Im(k z)
+    const double eta = dir == 0 ? ((s.x & 1) ? -1.0 : 1.0) : 1.0;
+    const int wraps = dir == 0 ? (bc.anti_t && s.t == G.nt - 1) : (bc.anti_x && s.x == G.nx - 1);
+    f += cg * 2.0 * (wraps ? -eta : eta) * k.y;
+    if (st && dir == 0 && ef != eb) {
+      const double2 cst = cjmul(st[i], st[in]);
+      f -= (ef - eb) * (k.x * cst.y + k.y * cst.x);
+    }
+    F[dir] = f;
+  }
+}
+
+// ------------------------------------------------ batched site operations --
+// One launch applies a per-site operation to every chain of a batch: the
+// host builds a table of per-chain descriptors (lattice, pointers,
+// scalars) with the first 256-site block of each chain; a block finds its
+// chain by binary search.
+struct BOp {
+  Geo G;
+  Bc bc;
+  long blk0;                   // first block of the chain
+  double s0, s1;               // scalars of the operation
+  double *d0, *d1, *d2, *d3, *d4;
+  float *f0;
+  double *out;                 // reduction result (device)
+};
+
+__device__ __forceinline__ int bop_chain(const BOp *ops, int nch, long blk)
+{
+  int lo = 0, hi = nch - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ops[mid].blk0 <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // --------------------------------------------------------- kernel entry --
 // Host-side launchers shared between translation units.
 namespace th2d {
@@ -184,4 +315,25 @@ int axpy_like(th2d_ctx *ctx, int op, long n, double a, double *y, const double *
               const double *z);
 int reduce_dot(th2d_ctx *ctx, long n_real, const double *a, const double *b, double *dev_out);
 int fetch_doubles(th2d_ctx *ctx, double *host, const double *dev, int n);
+
+// Batched site operations over a device table of nch descriptors covering
+// nblk blocks of 256 sites (table built by bop_finish).
+enum {
+  BOP_GAUGE_STEP,  // A(d0) += 2 s0 mom(d1); K(d2), K32(f0) refolded (f0 may be NULL)
+  BOP_FOLD,        // K(d2), K32(f0) from A(d0)
+  BOP_DSLASH,      // d0 = M d1 (s1 == 0) or M^dag d1 (s1 != 0), K = d2
+  BOP_FORCE,       // mom(d0) -= s0 F; K = d2, chi = d1, Mchi = d3, st = d4 (may be NULL); s1 = Nf/g
+  BOP_AXPY,        // d0 += s0 d1
+  BOP_SUB,         // d0 = d1 - d2
+  BOP_TO_F32,      // f0 = (float)(s0 d1)
+  BOP_ADD_F32,     // d0 += s0 f0
+  BOP_COPY,        // d0 = d1 (both link fields: 2 reals per direction)
+  BOP_SCALE,       // d0 = s0 d1
+  BOP_DOT,         // out = s1 * sum_sites Re(d0 . d1)  (two-pass, fixed order)
+  BOP_COSGAP,      // out = s1 * sum_links (1 - cos d0)
+};
+// Fills blk0 and returns the block count.
+long bop_finish(std::vector<BOp> &ops);
+int bop_upload(th2d_ctx *ctx, th2d_buf &dev, const std::vector<BOp> &ops);
+int bop_launch(th2d_ctx *ctx, int op, const BOp *dev, int nch, long nblk);
 }  // namespace th2d
