@@ -40,6 +40,7 @@ extern "C" {
 #define TH2D_ESINGULAR (-5)  /* LU met an exactly zero pivot                 */
 #define TH2D_ENOMEM (-6)     /* device allocation failed                     */
 #define TH2D_ELAUNCH (-7)    /* cooperative launch does not fit the device   */
+#define TH2D_ESYNC (-8)      /* a bounded wait between workgroups timed out  */
 
 #define TH2D_PREC_F64 0
 #define TH2D_PREC_F32 1
@@ -164,6 +165,15 @@ int th2d_set_cg_max_blocks(th2d_ctx *ctx, int n);
 int th2d_set_cg_autotune(th2d_ctx *ctx, int on);
 /* Tile shape used by the last solve on this context. */
 int th2d_cg_last_tile(th2d_ctx *ctx, int *tt, int *tx);
+/* Resident FP32 solver (1 on, default; 0 off): an FP32 inner solve whose
+ * lattices cut into no more 64x64 regions than the device holds resident
+ * workgroups (one per CU on MI355X: 256 regions, e.g. 16 lattices of
+ * 256^2) runs as one cooperative launch that keeps every region's vectors
+ * in registers; other solves use the step kernel.  A tile set with
+ * th2d_set_cg_tile also selects the step kernel.  th2d_cg_last_resident
+ * tells which kernel the last solve used (1 resident). */
+int th2d_set_cg_resident(th2d_ctx *ctx, int on);
+int th2d_cg_last_resident(th2d_ctx *ctx, int *on);
 /* Solver launches enqueued between two reads of the stop flags
  * (0 = the iterations the same restart pass took in the previous solve,
  * then 8 at a time). */

@@ -284,6 +284,78 @@ def test_batched_cg_uneven(hmc, oracle):
             assert np.linalg.norm(r) / np.linalg.norm(b) <= tol
 
 
+def _last_resident(ctx):
+    import ctypes as C
+    from thirring2d_amd import _native as N
+    on = C.c_int(-1)
+    assert N.lib().th2d_cg_last_resident(ctx.ptr, C.byref(on)) == 0
+    return on.value
+
+
+def test_cg_resident_matches_step_kernel(hmc, oracle):
+    """The resident FP32 solver (one cooperative launch, vectors in
+    registers, boundary layers exchanged through memory) and the step kernel
+    solve the same mixed-precision batch: regions cut by the lattice edge,
+    lattices smaller than a region and several regions per lattice; both
+    reach the true residual, with iteration counts within 10%."""
+    from thirring2d_amd import _native as N
+    dims = [(128, 96), (64, 64), (40, 72), (16, 16), (96, 200)]
+    setups = [_setup(hmc, oracle, nt, nx, 300 + k, m=0.05 + 0.03 * k)
+              for k, (nt, nx) in enumerate(dims)]
+    batch = hmc.ChainBatch([s[0] for s in setups])
+    ctx = setups[0][0].ctx
+    runs = {}
+    try:
+        for on in (1, 0):
+            assert N.lib().th2d_set_cg_resident(ctx.ptr, on) == 0
+            xs, infos = batch.fmdm_invert_cg([s[3] for s in setups],
+                                             hmc.cg_params(tol_rel=1e-10, precision=N.PREC_MIXED))
+            assert _last_resident(ctx) == on
+            runs[on] = (xs, infos)
+    finally:
+        N.lib().th2d_set_cg_resident(ctx.ptr, 1)
+    for k, (lat, L, A, b, _) in enumerate(setups):
+        for on in (1, 0):
+            x, inf = runs[on][0][k], runs[on][1][k]
+            assert inf.status == 0
+            r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
+            assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
+        it1, it0 = runs[1][1][k].iterations, runs[0][1][k].iterations
+        assert abs(it1 - it0) <= max(3, it0 // 10)
+
+
+def test_cg_resident_full_device(hmc, oracle):
+    """16 lattices of 256^2 are 256 regions, one per CU: the whole batch
+    runs resident and every chain reaches its true residual."""
+    from thirring2d_amd import _native as N
+    n = 16
+    setups = [_setup(hmc, oracle, 256, 256, 400 + k, m=0.05) for k in range(n)]
+    batch = hmc.ChainBatch([s[0] for s in setups])
+    xs, infos = batch.fmdm_invert_cg([s[3] for s in setups],
+                                     hmc.cg_params(tol_rel=1e-10, precision=N.PREC_MIXED))
+    assert _last_resident(setups[0][0].ctx) == 1
+    for k in range(n):
+        assert infos[k].status == 0
+        assert infos[k].true_rel_resid <= 1e-10
+    for k in (0, n - 1):
+        lat, L, A, b, _ = setups[k]
+        r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, xs[k]))
+        assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
+
+
+def test_cg_resident_falls_back_when_too_large(hmc, oracle):
+    """A batch with more regions than resident workgroups (5 x 512^2 = 320
+    regions) runs in the step kernel."""
+    from thirring2d_amd import _native as N
+    setups = [_setup(hmc, oracle, 512, 512, 500 + k, m=0.2) for k in range(5)]
+    batch = hmc.ChainBatch([s[0] for s in setups])
+    xs, infos = batch.fmdm_invert_cg([s[3] for s in setups],
+                                     hmc.cg_params(tol_rel=1e-10, precision=N.PREC_MIXED))
+    assert _last_resident(setups[0][0].ctx) == 0
+    for inf in infos:
+        assert inf.status == 0 and inf.true_rel_resid <= 1e-10
+
+
 def test_batched_trajectory_matches_oracle(hmc, oracle):
     """Chains of different sizes and couplings in one batched trajectory,
     each with its own MT noise: every chain equals the oracle's trajectory

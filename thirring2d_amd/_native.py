@@ -27,6 +27,7 @@ TH2D_EBREAKDOWN = -4
 TH2D_ESINGULAR = -5
 TH2D_ENOMEM = -6
 TH2D_ELAUNCH = -7
+TH2D_ESYNC = -8
 
 PREC_F64 = 0
 PREC_F32 = 1
@@ -113,6 +114,8 @@ SIGNATURES = {
     "th2d_set_cg_tile": (I, [P, I, I]),
     "th2d_cg_last_tile": (I, [P, C.POINTER(I), C.POINTER(I)]),
     "th2d_set_cg_autotune": (I, [P, I]),
+    "th2d_set_cg_resident": (I, [P, I]),
+    "th2d_cg_last_resident": (I, [P, C.POINTER(I)]),
     "th2d_set_dslash_variant": (I, [I]),
     "th2d_set_cg_max_blocks": (I, [P, I]),
     "th2d_set_cg_chunk": (I, [P, I]),

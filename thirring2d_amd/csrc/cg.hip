@@ -417,6 +417,335 @@ __global__ void __launch_bounds__(S::NT) k_cg_step(CgBatch<T> a, int slot)
   }
 }
 
+// ------------------------------------------------------ resident solver --
+// One workgroup per 64 x 64 region of a lattice for a whole inner solve,
+// every workgroup resident at once (cooperative launch).  The region's
+// r, w, s, p, x stay in registers and its folded links, with the 2-site
+// halo, in LDS, so an iteration reads and writes only the 2-site boundary
+// layers of r, s, w and one partial per region (the step kernel moves
+// 96 B per site and iteration in FP32).  Iteration j of a region:
+//   1. arrives on its chain's counter (iteration j-1 done) and waits until
+//      every region of the chain has;
+//   2. reads the chain's partials, and the halo layers of r_j, w_j, s_{j-1}
+//      that the neighbouring regions published;
+//   3. as the step kernel: s_j, r_{j+1} on region and halo, p_j, x_{j+1},
+//      y = M r_{j+1} on the 1-halo, w_{j+1} = M^dag y on the region;
+//   4. publishes its boundary layers of r_{j+1}, s_j, w_{j+1} and its
+//      partial of iteration j+1.
+// Published values are written and read with agent-scope relaxed atomics:
+// the stores are written through to the point of coherence of all XCDs
+// (no L2 write-back), and every wave waits for its stores before thread 0
+// arrives.  A site is published when it lies within 2 sites of its
+// region's edge; every halo value that reaches an owned site's result is
+// such a site (a halo site two steps from a region lies within two steps
+// of the edge of the region it belongs to).  All regions of a chain take
+// bitwise the same stop decision and leave together; every wait is bounded
+// (a wait that times out sets the error flag and ends its workgroup).
+using RShape = Shape<64, 64, 512>;
+constexpr int RES_NR = 4 * RShape::W2 + 4 * RShape::TT;  // halo elements of a region
+constexpr int RES_NQ = (RES_NR + RShape::NT - 1) / RShape::NT;
+constexpr unsigned RES_SPIN_LIMIT = 1u << 20;            // polls of one wait (about 1 s)
+
+struct ResCtl {
+  unsigned *bar;  // [nchains] arrival counters, zero at launch
+  int *err;       // set by a wait that timed out, zero at launch
+};
+
+__device__ __forceinline__ void pub_store(float2 *p, float2 v)
+{
+  __hip_atomic_store((unsigned long long *)p, __builtin_bit_cast(unsigned long long, v),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float2 pub_load(const float2 *p)
+{
+  return __builtin_bit_cast(float2, __hip_atomic_load((unsigned long long *)p, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ void pub_store(double *p, double v)
+{
+  __hip_atomic_store((unsigned long long *)p, __builtin_bit_cast(unsigned long long, v),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double pub_load(const double *p)
+{
+  return __builtin_bit_cast(double, __hip_atomic_load((unsigned long long *)p, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Element at byte offset `off` of a uniform base (the address mode with a
+// scalar base and a 32-bit vector offset; no 64-bit address per element).
+template <typename V> __device__ __forceinline__ V *at(V *base, unsigned off)
+{
+  return (V *)((char *)base + off);
+}
+template <typename V> __device__ __forceinline__ const V *at(const V *base, unsigned off)
+{
+  return (const V *)((const char *)base + off);
+}
+
+// Arrive on `bar` and wait until it reaches `target`; false when the wait
+// timed out here or elsewhere (then the error flag is set).
+__device__ bool res_sync(unsigned *bar, unsigned target, int *err, double *flag)
+{
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores have completed
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool ok = true;
+    for (unsigned n = 0;
+         __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; n++) {
+      if (n >= RES_SPIN_LIMIT ||
+          ((n & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = false;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    *flag = ok ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  return *flag != 0.0;
+}
+
+// Position (row, column) in the (TT+4) x (TX+4) halo frame of halo element
+// q: two rows above, two below, then two columns left and right.
+__device__ __forceinline__ void res_ring_pos(int q, int &et, int &ex)
+{
+  constexpr int W2 = RShape::W2, TT = RShape::TT, TX = RShape::TX;
+  if (q < 2 * W2) {
+    et = q / W2;
+    ex = q - et * W2;
+  } else if (q < 4 * W2) {
+    q -= 2 * W2;
+    et = TT + 2 + q / W2;
+    ex = q - (q / W2) * W2;
+  } else {
+    q -= 4 * W2;
+    const int side = q / (2 * TT), k = q - side * 2 * TT;
+    et = 2 + (k >> 1);
+    ex = (k & 1) + (side ? TX + 2 : 0);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
+{
+  using S = RShape;
+  using V2 = c2<T>;
+  constexpr int TT = S::TT, TX = S::TX, NT = S::NT, W2 = S::W2, W1 = S::W1;
+  constexpr int E2 = S::E2, E1 = S::E1, NO = S::NO, N1 = S::N1;
+  constexpr int LDS_D = (3 * E2 + E1) * (int)sizeof(V2) / 8;
+  __shared__ double rlds[LDS_D + 40];
+  V2 *R2 = (V2 *)rlds, *Y1 = R2 + E2, *LKt = Y1 + E1, *LKx = LKt + E2;
+  double *red = rlds + LDS_D;
+  const int tid = threadIdx.x, tau = blockIdx.x;
+  const CgTile tl = a.tiles[tau];
+  const CgChain ch = a.chains[tl.chain];
+  const Geo G = ch.G;
+  const int V = 2 * G.vh, ntc = ch.tend - ch.tbeg;
+  const bool first = tau == ch.tbeg;
+  const int tte = min(TT, G.nt - tl.t0), txe = min(TX, G.nx - tl.x0);
+  const V2 *K = (const V2 *)ch.K, *b = (const V2 *)ch.b;
+  unsigned *bar = ctl.bar + tl.chain;
+  const T m = (T)G.m, epm = (T)G.ep, emm = (T)G.em;
+
+  // links and r_0 = b on the region and its halo
+  for (int e = tid; e < E2; e += NT) {
+    const int et = e / W2, ex = e - et * W2;
+    const int i = eo_index(G, wrapn(tl.t0 - 2 + et, G.nt), wrapn(tl.x0 - 2 + ex, G.nx));
+    LKt[e] = K[i];
+    LKx[e] = K[V + i];
+    R2[e] = b[i];
+  }
+  // owned elements e = tid + q NT of the region; those outside the lattice
+  // (a region larger than what is left of it) are halo, reloaded each time
+  unsigned oi[NO];  // byte offsets of the owned elements in a chain vector
+  unsigned inl = 0, pub = 0;
+  V2 r[NO], w[NO], s[NO], p[NO], x[NO];
+  const V2 zero = cmk<V2>((T)0, (T)0);
+#pragma unroll
+  for (int q = 0; q < NO; q++) {
+    const int e = tid + q * NT, lt = e / TX, lx = e - lt * TX;
+    oi[q] = (unsigned)eo_index(G, wrapn(tl.t0 + lt, G.nt), wrapn(tl.x0 + lx, G.nx)) *
+            (unsigned)sizeof(V2);
+    if (lt < tte && lx < txe) {
+      inl |= 1u << q;
+      if (lt < 2 || lt >= tte - 2 || lx < 2 || lx >= txe - 2) pub |= 1u << q;
+    }
+    r[q] = *at(b, oi[q]);
+    w[q] = s[q] = p[q] = x[q] = zero;
+  }
+  unsigned ri[RES_NQ];  // byte offsets of the halo elements
+  int re[RES_NQ];
+#pragma unroll
+  for (int h = 0; h < RES_NQ; h++) {
+    ri[h] = re[h] = 0;
+    if (tid + h * NT < RES_NR) {
+      int et, ex;
+      res_ring_pos(tid + h * NT, et, ex);
+      re[h] = et * W2 + ex;
+      ri[h] = (unsigned)eo_index(G, wrapn(tl.t0 - 2 + et, G.nt), wrapn(tl.x0 - 2 + ex, G.nx)) *
+              (unsigned)sizeof(V2);
+    }
+  }
+  V2 *const rb0 = a.r0 + ch.voff, *const rb1 = a.r1 + ch.voff;
+  V2 *const sb0 = a.s0 + ch.voff, *const sb1 = a.s1 + ch.voff;
+  V2 *const wb0 = a.w0 + ch.voff, *const wb1 = a.w1 + ch.voff;
+  double gam_old = 1.0, alpha_old = 1.0;
+  int slot = 0;
+  // iteration -1 (set-up): w_0 = A b; T ta = 0 marks it
+  bool setup = true;
+  T ta = 0, tb = 0;
+  for (int j = -1;; j++) {
+    V2 *const rn_ = slot ? rb0 : rb1, *const sn_ = slot ? sb0 : sb1, *const wn_ = slot ? wb0 : wb1;
+    if (!setup) {
+      // 3a. s_j, r_{j+1} on the halo and the region; p_j, x_{j+1}
+      const V2 *rc = slot ? rb1 : rb0, *wc = slot ? wb1 : wb0, *sc = slot ? sb1 : sb0;
+#pragma unroll
+      for (int h = 0; h < RES_NQ; h++) {
+        if (tid + h * NT >= RES_NR) continue;
+        const V2 hr = pub_load(at(rc, ri[h])), hw = pub_load(at(wc, ri[h]));
+        const V2 hs = pub_load(at(sc, ri[h]));
+        const V2 sn = cadd(hw, cscale(hs, tb));
+        R2[re[h]] = csub(hr, cscale(sn, ta));
+      }
+#pragma unroll
+      for (int q = 0; q < NO; q++) {
+        const int e = tid + q * NT, lt = e / TX, lx = e - lt * TX;
+        const bool own = (inl >> q) & 1u;
+        if (!own) {
+          r[q] = pub_load(at(rc, oi[q]));
+          w[q] = pub_load(at(wc, oi[q]));
+          s[q] = pub_load(at(sc, oi[q]));
+        }
+        const V2 sn = cadd(w[q], cscale(s[q], tb));
+        const V2 rn = csub(r[q], cscale(sn, ta));
+        R2[(lt + 2) * W2 + (lx + 2)] = rn;
+        if (own) {
+          p[q] = cadd(r[q], cscale(p[q], tb));
+          x[q] = cadd(x[q], cscale(p[q], ta));
+        }
+        r[q] = rn;
+        s[q] = sn;
+        if ((pub >> q) & 1u) {
+          pub_store(at(rn_, oi[q]), rn);
+          pub_store(at(sn_, oi[q]), sn);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < NO; q++)
+        if ((pub >> q) & 1u) {
+          pub_store(at(rb0, oi[q]), r[q]);
+          pub_store(at(sb0, oi[q]), zero);
+        }
+    }
+    __syncthreads();
+    // 3b. y = M r_{j+1} on the 1-halo (LDS only; not unrolled, which
+    // would hoist every element's loads into registers)
+#pragma unroll 1
+    for (int q = 0; q < N1; q++) {
+      const int e = tid + q * NT;
+      if (!(E1 % NT == 0 || e < E1)) continue;
+      const int ft = e / W1, fx = e - ft * W1;
+      const int c = (ft + 1) * W2 + (fx + 1);
+      const V2 kt = LKt[c], kx = LKx[c], ktd = LKt[c - W2], kxd = LKx[c - 1];
+      V2 y = cscale(R2[c], m);
+      y = cadd(y, cscale(cmul(kt, R2[c + W2]), epm));
+      y = csub(y, cscale(cjmul(ktd, R2[c - W2]), emm));
+      y = cadd(y, cmul(kx, R2[c + 1]));
+      y = csub(y, cjmul(kxd, R2[c - 1]));
+      Y1[e] = y;
+    }
+    __syncthreads();
+    // 3c. w_{j+1} = M^dag y on the region; partials of iteration j+1
+    double gacc = 0.0, dacc = 0.0;
+#pragma unroll
+    for (int q = 0; q < NO; q++) {
+      if (!((inl >> q) & 1u)) continue;
+      const int e = tid + q * NT, lt = e / TX, lx = e - lt * TX;
+      const int c1 = (lt + 1) * W1 + (lx + 1);
+      const int c2i = (lt + 2) * W2 + (lx + 2);
+      const V2 kt = LKt[c2i], kx = LKx[c2i], ktd = LKt[c2i - W2], kxd = LKx[c2i - 1];
+      V2 v = cscale(Y1[c1], m);
+      v = csub(v, cscale(cmul(kt, Y1[c1 + W1]), emm));
+      v = cadd(v, cscale(cjmul(ktd, Y1[c1 - W1]), epm));
+      v = csub(v, cmul(kx, Y1[c1 + 1]));
+      v = cadd(v, cjmul(kxd, Y1[c1 - 1]));
+      w[q] = v;
+      if ((pub >> q) & 1u) pub_store(at(setup ? wb0 : wn_, oi[q]), v);
+      gacc += (double)r[q].x * (double)r[q].x + (double)r[q].y * (double)r[q].y;
+      dacc += (double)r[q].x * (double)v.x + (double)r[q].y * (double)v.y;
+      __builtin_amdgcn_sched_barrier(0);  // one element's loads in flight at a time
+    }
+    // 4. the region's partial (fixed order: lanes, then waves)
+    gacc = wave_sum(gacc);
+    dacc = wave_sum(dacc);
+    if ((tid & 63) == 0) {
+      red[2 * (tid >> 6)] = gacc;
+      red[2 * (tid >> 6) + 1] = dacc;
+    }
+    __syncthreads();
+    const int nslot = setup ? 0 : slot ^ 1;
+    if (tid == 0) {
+      double g = 0.0, d = 0.0;
+      for (int k = 0; k < NT / 64; k++) {
+        g += red[2 * k];
+        d += red[2 * k + 1];
+      }
+      pub_store(a.part + ((long)nslot * a.ntiles + tau) * 2 + 0, g);
+      pub_store(a.part + ((long)nslot * a.ntiles + tau) * 2 + 1, d);
+    }
+    slot = nslot;
+    setup = false;
+    // 1. every region of the chain has finished iteration j
+    if (!res_sync(bar, (unsigned)(j + 2) * (unsigned)ntc, ctl.err, red + 36)) break;
+    // 2. the chain's partials of iteration j+1 (same order in every region)
+    if (tid < 64) {
+      const double *pp = a.part + (long)slot * a.ntiles * 2;
+      double pg = 0.0, pd = 0.0;
+      for (int k = ch.tbeg + tid; k < ch.tend; k += 64) {
+        pg += pub_load(pp + 2 * k);
+        pd += pub_load(pp + 2 * k + 1);
+      }
+      pg = wave_sum(pg);
+      pd = wave_sum(pd);
+      if (tid == 0) {
+        red[32] = pg;
+        red[33] = pd;
+      }
+    }
+    __syncthreads();
+    const double gam = red[32], del = red[33];
+    const int jn = j + 1;
+    int status = 1;  // 1: continue
+    double alpha = 0.0, beta = 0.0;
+    if (gam < ch.stop_rr) status = TH2D_OK;
+    else if (jn >= ch.max_iter) status = TH2D_ENOCONV;
+    else if (!(del > 0.0) || !isfinite(gam)) status = TH2D_EBREAKDOWN;
+    else {
+      beta = jn == 0 ? 0.0 : gam / gam_old;
+      alpha = jn == 0 ? gam / del : gam / (del - beta * gam / alpha_old);
+      if (!(alpha > 0.0) || !isfinite(alpha)) status = TH2D_EBREAKDOWN;
+    }
+    if (status != 1) {
+      if (first && tid == 0) a.res[tl.chain] = CgResult{jn, status, gam, 1, 0};
+      break;
+    }
+    gam_old = gam;
+    alpha_old = alpha;
+    ta = (T)alpha;
+    tb = (T)beta;
+    // red[32..33] are rewritten only after the next two barriers
+  }
+  // x of the owned sites
+  V2 *xo = a.x + ch.voff;
+#pragma unroll
+  for (int q = 0; q < NO; q++)
+    if ((inl >> q) & 1u) *at(xo, oi[q]) = x[q];
+}
+
 // Default tile shape for a batch: among the compiled shapes no larger
 // than any lattice of the batch, the one with the least modelled load
 // cost, loaded elements per owned site (2-site halo and padding of
@@ -574,12 +903,191 @@ int autotune_shape(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2
   return best >= 0 ? best : TH2D_EARG;
 }
 
+// Chain and tile tables of a batch cut into tt x tx tiles, and the device
+// block of the solver (8 vectors, partials, state, results, tables and
+// `extra` bytes), in `buf`; the tables are uploaded on the context stream.
+template <typename T>
+int build_batch(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T> *const *rhs,
+                const c2<T> *const *K, const double *stop_rr, const int *max_iter, int tt, int tx,
+                size_t extra, CgBatch<T> &a, std::vector<CgChain> &ch, void **extra_ptr)
+{
+  ch.assign(nch, CgChain{});
+  std::vector<CgTile> tiles;
+  long voff = 0;
+  for (int c = 0; c < nch; c++) {
+    ch[c].G = G[c];
+    ch[c].tbeg = (int)tiles.size();
+    for (int t0 = 0; t0 < G[c].nt; t0 += tt)
+      for (int x0 = 0; x0 < G[c].nx; x0 += tx) tiles.push_back(CgTile{c, t0, x0, 0});
+    ch[c].tend = (int)tiles.size();
+    ch[c].voff = voff;
+    voff += (2L * G[c].vh + 63) & ~63L;
+    ch[c].b = rhs[c];
+    ch[c].K = K[c];
+    ch[c].stop_rr = stop_rr[c];
+    ch[c].max_iter = max_iter[c];
+    ch[c].pad = 0;
+  }
+  const int ntiles = (int)tiles.size();
+  size_t vec = sizeof(c2<T>) * (size_t)voff;
+  vec = (vec + 255) & ~(size_t)255;
+  auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+  size_t part = al(sizeof(double) * 4 * (size_t)ntiles);
+  size_t stb = al(sizeof(CgState) * 2 * (size_t)nch);
+  size_t rsb = al(sizeof(CgResult) * (size_t)nch);
+  size_t chb = al(sizeof(CgChain) * (size_t)nch);
+  size_t tlb = al(sizeof(CgTile) * (size_t)ntiles);
+  size_t exb = al(extra);
+  size_t bytes = 8 * vec + part + stb + rsb + chb + tlb + exb;
+  TH2D_TRY(th2d_buf_reserve(buf, bytes));
+  char *pb = (char *)buf.ptr;
+  c2<T> **vp[8] = {&a.x, &a.p, &a.r0, &a.r1, &a.s0, &a.s1, &a.w0, &a.w1};
+  for (int k = 0; k < 8; k++) {
+    *vp[k] = (c2<T> *)pb;
+    pb += vec;
+  }
+  a.part = (double *)pb;
+  pb += part;
+  a.state = (CgState *)pb;
+  pb += stb;
+  a.res = (CgResult *)pb;
+  pb += rsb;
+  CgChain *dch = (CgChain *)pb;
+  pb += chb;
+  CgTile *dtl = (CgTile *)pb;
+  pb += tlb;
+  *extra_ptr = pb;
+  pb += exb;
+  if ((size_t)(pb - (char *)buf.ptr) > bytes) return TH2D_EARG;
+  a.chains = dch;
+  a.tiles = dtl;
+  a.ntiles = ntiles;
+  a.nchains = nch;
+  a.TT = tt;
+  a.TX = tx;
+  // tables through a pinned staging block
+  size_t hbytes = std::max(chb + tlb, rsb);
+  TH2D_TRY(th2d_host_reserve(ctx->redh, hbytes + 64));
+  memcpy(ctx->redh.ptr, ch.data(), sizeof(CgChain) * nch);
+  memcpy((char *)ctx->redh.ptr + chb, tiles.data(), sizeof(CgTile) * ntiles);
+  TH2D_HIP(hipMemcpyAsync(dch, ctx->redh.ptr, chb + tlb, hipMemcpyHostToDevice, ctx->stream));
+  return TH2D_OK;
+}
+
+// Per-chain results into the caller's arrays, the solver's kernel time and
+// algorithmic bytes into the context counters.
+template <typename T>
+int finish_solve(th2d_ctx *ctx, int nch, const Geo *G, const CgBatch<T> &a,
+                 const std::vector<CgChain> &ch, const std::vector<CgResult> &h, long launches,
+                 c2<T> **xout, InnerResult *res)
+{
+  if (ctx->ktiming) {
+    TH2D_HIP(hipEventRecord(ctx->k1, ctx->stream));
+    TH2D_HIP(hipEventSynchronize(ctx->k1));
+    float ms = 0.f;
+    TH2D_HIP(hipEventElapsedTime(&ms, ctx->k0, ctx->k1));
+    ctx->ktime_ms += ms;
+    ctx->klaunches += launches;
+    // algorithmic bytes of one CG iteration per site: two operator
+    // applications (in + out + 2 links each: 8 complex) and the fused
+    // vector updates (7 complex): 240 B in FP64, 120 B in FP32.
+    double vb = sizeof(c2<T>);
+    for (int c = 0; c < nch; c++) {
+      ctx->kbytes += (double)h[c].iters * 2.0 * G[c].vh * 15.0 * vb;
+      ctx->kiters += h[c].iters;
+    }
+  }
+  for (int c = 0; c < nch; c++) {
+    res[c].iters = h[c].iters;
+    res[c].rr = h[c].rr;
+    res[c].status = h[c].status;
+    xout[c] = a.x + ch[c].voff;
+  }
+  return TH2D_OK;
+}
+
+__global__ void k_zero_u32(unsigned *p, int n)
+{
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// Regions of the resident solver for a batch.
+int res_regions(int nch, const Geo *G)
+{
+  long n = 0;
+  for (int c = 0; c < nch; c++)
+    n += (long)((G[c].nt + RShape::TT - 1) / RShape::TT) * ((G[c].nx + RShape::TX - 1) / RShape::TX);
+  return n > (1L << 30) ? (1 << 30) : (int)n;
+}
+
+// Resident workgroups the device holds (cooperative launch bound).
+int res_capacity(th2d_ctx *ctx)
+{
+  if (ctx->res_capacity < 0) {
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)k_cg_resident<float>,
+                                                     RShape::NT, 0) != hipSuccess)
+      occ = 0;
+    ctx->res_capacity = occ * ctx->num_cu;
+  }
+  return ctx->res_capacity;
+}
+
+// One inner solve as a single cooperative launch of the resident kernel
+// (FP32); TH2D_ELAUNCH when the batch has more regions than the device
+// holds resident workgroups.
+int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
+                   const c2<float> *const *rhs, const c2<float> *const *K, const double *stop_rr,
+                   const int *max_iter, c2<float> **xout, InnerResult *res)
+{
+  const int nreg = res_regions(nch, G);
+  if (nreg < 1 || nreg > res_capacity(ctx)) return TH2D_ELAUNCH;
+  CgBatch<float> a;
+  std::vector<CgChain> ch;
+  void *extra = nullptr;
+  TH2D_TRY(build_batch<float>(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, RShape::TT,
+                              RShape::TX, sizeof(unsigned) * (nch + 1), a, ch, &extra));
+  if (a.ntiles != nreg) return TH2D_EARG;
+  ResCtl ctl;
+  ctl.bar = (unsigned *)extra;
+  ctl.err = (int *)(ctl.bar + nch);
+  ctx->cg_last_tile[0] = RShape::TT;
+  ctx->cg_last_tile[1] = RShape::TX;
+  ctx->cg_last_resident = 1;
+  if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k0, ctx->stream));
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, ctx->stream, ctl.bar, nch + 1);
+  TH2D_HIP(hipGetLastError());
+  void *args[] = {(void *)&a, (void *)&ctl};
+  TH2D_HIP(hipLaunchCooperativeKernel(k_cg_resident<float>, dim3(nreg), dim3(RShape::NT), args, 0,
+                                      ctx->stream));
+  // results and the error flag in one copy
+  const size_t rb = sizeof(CgResult) * nch;
+  TH2D_TRY(th2d_host_reserve(ctx->redh, rb + 64));
+  TH2D_HIP(hipMemcpyAsync(ctx->redh.ptr, a.res, rb, hipMemcpyDeviceToHost, ctx->stream));
+  TH2D_HIP(hipMemcpyAsync((char *)ctx->redh.ptr + rb, ctl.err, sizeof(int), hipMemcpyDeviceToHost,
+                          ctx->stream));
+  TH2D_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<CgResult> h(nch);
+  memcpy(h.data(), ctx->redh.ptr, rb);
+  int err = 0;
+  memcpy(&err, (char *)ctx->redh.ptr + rb, sizeof(int));
+  if (err) return TH2D_ESYNC;
+  return finish_solve<float>(ctx, nch, G, a, ch, h, 1, xout, res);
+}
+
 template <typename T>
 int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
                 const c2<T> *const *rhs,
                 const c2<T> *const *K, const double *stop_rr, const int *max_iter, c2<T> **xout,
                 InnerResult *res, int force_shape)
 {
+  if constexpr (sizeof(T) == 4) {
+    if (force_shape < 0 && ctx->cg_resident && !(ctx->tile_t > 0 && ctx->tile_x > 0)) {
+      const int rc = resident_solve(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, xout, res);
+      if (rc != TH2D_ELAUNCH) return rc;
+    }
+  }
+  ctx->cg_last_resident = 0;
   int shape = force_shape;
   if (shape < 0 && ctx->cg_autotune && !(ctx->tile_t > 0 && ctx->tile_x > 0)) {
     auto &cache = ctx->cg_tuned[sizeof(T) == 8 ? 1 : 0];
@@ -602,66 +1110,12 @@ int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
   const int nthreads = tc.nt;
   ctx->cg_last_tile[0] = tc.tt;
   ctx->cg_last_tile[1] = tc.tx;
-  // host tables
-  std::vector<CgChain> ch(nch);
-  std::vector<CgTile> tiles;
-  long voff = 0;
-  for (int c = 0; c < nch; c++) {
-    ch[c].G = G[c];
-    ch[c].tbeg = (int)tiles.size();
-    for (int t0 = 0; t0 < G[c].nt; t0 += tc.tt)
-      for (int x0 = 0; x0 < G[c].nx; x0 += tc.tx) tiles.push_back(CgTile{c, t0, x0, 0});
-    ch[c].tend = (int)tiles.size();
-    ch[c].voff = voff;
-    voff += (2L * G[c].vh + 63) & ~63L;
-    ch[c].b = rhs[c];
-    ch[c].K = K[c];
-    ch[c].stop_rr = stop_rr[c];
-    ch[c].max_iter = max_iter[c];
-    ch[c].pad = 0;
-  }
-  const int ntiles = (int)tiles.size();
-  // device block: 8 vectors, partials, state, results, chain and tile tables
-  size_t vec = sizeof(c2<T>) * (size_t)voff;
-  vec = (vec + 255) & ~(size_t)255;
-  auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
-  size_t part = al(sizeof(double) * 4 * (size_t)ntiles);
-  size_t stb = al(sizeof(CgState) * 2 * (size_t)nch);
-  size_t rsb = al(sizeof(CgResult) * (size_t)nch);
-  size_t chb = al(sizeof(CgChain) * (size_t)nch);
-  size_t tlb = al(sizeof(CgTile) * (size_t)ntiles);
-  size_t bytes = 8 * vec + part + stb + rsb + chb + tlb;
-  TH2D_TRY(th2d_buf_reserve(buf, bytes));
-  char *pb = (char *)buf.ptr;
   CgBatch<T> a;
-  c2<T> **vp[8] = {&a.x, &a.p, &a.r0, &a.r1, &a.s0, &a.s1, &a.w0, &a.w1};
-  for (int k = 0; k < 8; k++) {
-    *vp[k] = (c2<T> *)pb;
-    pb += vec;
-  }
-  a.part = (double *)pb;
-  pb += part;
-  a.state = (CgState *)pb;
-  pb += stb;
-  a.res = (CgResult *)pb;
-  pb += rsb;
-  CgChain *dch = (CgChain *)pb;
-  pb += chb;
-  CgTile *dtl = (CgTile *)pb;
-  pb += tlb;
-  if ((size_t)(pb - (char *)buf.ptr) > bytes) return TH2D_EARG;
-  a.chains = dch;
-  a.tiles = dtl;
-  a.ntiles = ntiles;
-  a.nchains = nch;
-  a.TT = tc.tt;
-  a.TX = tc.tx;
-  // tables through a pinned staging block
-  size_t hbytes = std::max(chb + tlb, rsb);
-  TH2D_TRY(th2d_host_reserve(ctx->redh, hbytes + 64));
-  memcpy(ctx->redh.ptr, ch.data(), sizeof(CgChain) * nch);
-  memcpy((char *)ctx->redh.ptr + chb, tiles.data(), sizeof(CgTile) * ntiles);
-  TH2D_HIP(hipMemcpyAsync(dch, ctx->redh.ptr, chb + tlb, hipMemcpyHostToDevice, ctx->stream));
+  std::vector<CgChain> ch;
+  void *extra = nullptr;
+  TH2D_TRY(build_batch<T>(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, tc.tt, tc.tx, 0, a, ch,
+                          &extra));
+  const int ntiles = a.ntiles;
   const size_t lds = lds_bytes<T>(tc.tt, tc.tx);
   // one block per tile unless capped; a capped grid is a multiple of 8 so
   // that a block keeps its XCD over the grid-stride loop
@@ -701,29 +1155,7 @@ int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
   int itmax = 0;
   for (int c = 0; c < nch; c++) itmax = std::max(itmax, h[c].iters);
   last = itmax;
-  if (ctx->ktiming) {
-    TH2D_HIP(hipEventRecord(ctx->k1, ctx->stream));
-    TH2D_HIP(hipEventSynchronize(ctx->k1));
-    float ms = 0.f;
-    TH2D_HIP(hipEventElapsedTime(&ms, ctx->k0, ctx->k1));
-    ctx->ktime_ms += ms;
-    ctx->klaunches += launched + 1;
-    // algorithmic bytes of one CG iteration per site: two operator
-    // applications (in + out + 2 links each: 8 complex) and the fused
-    // vector updates (7 complex): 240 B in FP64, 120 B in FP32.
-    double vb = sizeof(c2<T>);
-    for (int c = 0; c < nch; c++) {
-      ctx->kbytes += (double)h[c].iters * 2.0 * G[c].vh * 15.0 * vb;
-      ctx->kiters += h[c].iters;
-    }
-  }
-  for (int c = 0; c < nch; c++) {
-    res[c].iters = h[c].iters;
-    res[c].rr = h[c].rr;
-    res[c].status = h[c].status;
-    xout[c] = a.x + ch[c].voff;
-  }
-  return TH2D_OK;
+  return finish_solve<T>(ctx, nch, G, a, ch, h, launched + 1, xout, res);
 }
 
 
@@ -744,6 +1176,20 @@ extern "C" int th2d_set_cg_autotune(th2d_ctx *ctx, int on)
 {
   if (!ctx) return TH2D_EARG;
   ctx->cg_autotune = on ? 1 : 0;
+  return TH2D_OK;
+}
+
+extern "C" int th2d_set_cg_resident(th2d_ctx *ctx, int on)
+{
+  if (!ctx) return TH2D_EARG;
+  ctx->cg_resident = on ? 1 : 0;
+  return TH2D_OK;
+}
+
+extern "C" int th2d_cg_last_resident(th2d_ctx *ctx, int *on)
+{
+  if (!ctx || !on) return TH2D_EARG;
+  *on = ctx->cg_last_resident;
   return TH2D_OK;
 }
 

@@ -97,6 +97,7 @@ extern "C" const char *th2d_strerror(int code)
     case TH2D_ESINGULAR: return "singular matrix (zero pivot)";
     case TH2D_ENOMEM: return "device allocation failed";
     case TH2D_ELAUNCH: return "cooperative launch does not fit the device";
+    case TH2D_ESYNC: return "a bounded wait between workgroups timed out";
     default: return "unknown th2d status";
   }
 }
