@@ -66,7 +66,8 @@ struct CgChain {
   const void *b;    // right-hand side (eo, precision T)
   const void *K;    // folded links ([2][V], precision T)
   double stop_rr;
-  int max_iter, pad;
+  int max_iter;
+  int pad;          // resident solver: first partial slot of the chain (8-aligned)
 };
 
 struct CgTile {
@@ -432,10 +433,11 @@ __global__ void __launch_bounds__(S::NT) k_cg_step(CgBatch<T> a, int slot)
 //      y = M r_{j+1} on the 1-halo, w_{j+1} = M^dag y on the region;
 //   4. publishes its boundary layers of r_{j+1}, s_j, w_{j+1} and its
 //      partial of iteration j+1.
-// Published values are written and read with agent-scope relaxed atomics:
-// the stores are written through to the point of coherence of all XCDs
-// (no L2 write-back), and every wave waits for its stores before thread 0
-// arrives.  A site is published when it lies within 2 sites of its
+// Regions are assigned to workgroups XCD by XCD, so a lattice of up to
+// ntiles/8 regions runs on one XCD; such a chain exchanges through that
+// XCD's L2, the others through the point of coherence of all XCDs (see
+// ResMem).  Every wave waits for its stores before thread 0 arrives.
+// A site is published when it lies within 2 sites of its
 // region's edge; every halo value that reaches an owned site's result is
 // such a site (a halo site two steps from a region lies within two steps
 // of the edge of the region it belongs to).  All regions of a chain take
@@ -447,8 +449,15 @@ constexpr int RES_NQ = (RES_NR + RShape::NT - 1) / RShape::NT;
 constexpr unsigned RES_SPIN_LIMIT = 1u << 20;            // polls of one wait (about 1 s)
 
 struct ResCtl {
-  unsigned *bar;  // [nchains] arrival counters, zero at launch
-  int *err;       // set by a wait that timed out, zero at launch
+  unsigned *bar;     // [nchains][32] arrival counters of the iterations (own cache line each)
+  unsigned *bar0;    // [nchains][32] set-up counters (agent scope)
+  int *err;          // set by a wait that timed out
+  int *xcc;          // [ntiles] XCD of the workgroup of each region
+  double *part;      // [2][nparts][2] partials; a chain's start on a 128-B boundary
+  int nparts;        // partial slots per parity
+  unsigned blk_bytes;  // bytes of the solver block from CgBatch::x (buffer range)
+  unsigned zero;     // 0; an addend the compiler cannot see, so that a poll
+                     // stays a read-modify-write atomic (executed in L2)
 };
 
 __device__ __forceinline__ void pub_store(float2 *p, float2 v)
@@ -472,8 +481,7 @@ __device__ __forceinline__ double pub_load(const double *p)
                                                       __HIP_MEMORY_SCOPE_AGENT));
 }
 
-// Element at byte offset `off` of a uniform base (the address mode with a
-// scalar base and a 32-bit vector offset; no 64-bit address per element).
+// Element at byte offset `off` of a base pointer (a 32-bit offset per lane).
 template <typename V> __device__ __forceinline__ V *at(V *base, unsigned off)
 {
   return (V *)((char *)base + off);
@@ -483,30 +491,83 @@ template <typename V> __device__ __forceinline__ const V *at(const V *base, unsi
   return (const V *)((const char *)base + off);
 }
 
-// Arrive on `bar` and wait until it reaches `target`; false when the wait
-// timed out here or elsewhere (then the error flag is set).
-__device__ bool res_sync(unsigned *bar, unsigned target, int *err, double *flag)
+// Invalidates this CU's vector L1, so that the next loads read L2.
+__device__ __forceinline__ void l1_invalidate()
 {
-  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores have completed
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    bool ok = true;
-    for (unsigned n = 0;
-         __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; n++) {
-      if (n >= RES_SPIN_LIMIT ||
-          ((n & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ok = false;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-    *flag = ok ? 1.0 : 0.0;
-  }
-  __syncthreads();
-  return *flag != 0.0;
+  __asm__ volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
 }
+
+// Memory of the resident solver: the solver block at `base`, addressed by
+// a uniform byte offset plus a per-lane byte offset.  A chain whose
+// regions all run on one XCD exchanges through that XCD's L2 ("local"):
+// plain stores (the vector L1 writes through to L2), L2 atomics for the
+// arrival counter, polled with a returning read-modify-write (a plain or
+// scope-0 load of the counter may be served by a stale L1 line, and the
+// compiler turns an atomic add of a known 0 into such a load), then an L1
+// invalidation before the reads.  Other chains use agent-scope relaxed
+// atomics (written through to the point of coherence of all XCDs, read
+// from there).
+struct ResMem {
+  char *base;
+  __amdgpu_buffer_rsrc_t rs;
+  bool local;
+
+  __device__ __forceinline__ float2 ld2(unsigned soff, unsigned voff) const
+  {
+    if (local)
+      return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 1));
+    return pub_load((const float2 *)(base + soff + voff));
+  }
+  __device__ __forceinline__ void st2(unsigned soff, unsigned voff, float2 v) const
+  {
+    float2 *p = (float2 *)(base + soff + voff);
+    if (local) *p = v;
+    else pub_store(p, v);
+  }
+  __device__ __forceinline__ double ldd(unsigned soff, unsigned voff) const
+  {
+    if (local)
+      return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 1));
+    return pub_load((const double *)(base + soff + voff));
+  }
+  __device__ __forceinline__ void std_(unsigned soff, unsigned voff, double v) const
+  {
+    double *p = (double *)(base + soff + voff);
+    if (local) *p = v;
+    else pub_store(p, v);
+  }
+  // Arrive on the counter at byte offset `obar` and wait until it reaches
+  // `target`; false when the wait timed out here or elsewhere (then the
+  // error flag is set).  Every wave first waits for its own stores.
+  __device__ bool sync(unsigned obar, unsigned target, int *err, double *flag, unsigned zero) const
+  {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores have completed
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned *bar = (unsigned *)(base + obar);
+      if (local) __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      else __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      bool ok = true;
+      for (unsigned n = 0;; n++) {
+        const unsigned v =
+            local ? __hip_atomic_fetch_add(bar, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                  : __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v >= target) break;
+        if (n >= RES_SPIN_LIMIT ||
+            ((n & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = false;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (local) l1_invalidate();  // one L1 per CU, and this workgroup is alone on it
+      *flag = ok ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    return *flag != 0.0;
+  }
+};
 
 // Position (row, column) in the (TT+4) x (TX+4) halo frame of halo element
 // q: two rows above, two below, then two columns left and right.
@@ -528,18 +589,27 @@ __device__ __forceinline__ void res_ring_pos(int q, int &et, int &ex)
   }
 }
 
+// XCD of the calling workgroup (hardware register XCC_ID, bits 3:0).
+__device__ __forceinline__ int xcc_id()
+{
+  return (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
 {
   using S = RShape;
   using V2 = c2<T>;
+  static_assert(sizeof(V2) == 8, "the resident solver is FP32");
   constexpr int TT = S::TT, TX = S::TX, NT = S::NT, W2 = S::W2, W1 = S::W1;
   constexpr int E2 = S::E2, E1 = S::E1, NO = S::NO, N1 = S::N1;
   constexpr int LDS_D = (3 * E2 + E1) * (int)sizeof(V2) / 8;
   __shared__ double rlds[LDS_D + 40];
   V2 *R2 = (V2 *)rlds, *Y1 = R2 + E2, *LKt = Y1 + E1, *LKx = LKt + E2;
   double *red = rlds + LDS_D;
-  const int tid = threadIdx.x, tau = blockIdx.x;
+  const int tid = threadIdx.x;
+  // regions XCD by XCD: a lattice of up to ntiles/8 regions runs on one XCD
+  const int tau = tile_of(blockIdx.x, a.ntiles);
   const CgTile tl = a.tiles[tau];
   const CgChain ch = a.chains[tl.chain];
   const Geo G = ch.G;
@@ -547,8 +617,20 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   const bool first = tau == ch.tbeg;
   const int tte = min(TT, G.nt - tl.t0), txe = min(TX, G.nx - tl.x0);
   const V2 *K = (const V2 *)ch.K, *b = (const V2 *)ch.b;
-  unsigned *bar = ctl.bar + tl.chain;
   const T m = (T)G.m, epm = (T)G.ep, emm = (T)G.em;
+  ResMem M;
+  M.base = (char *)a.x;
+  M.rs = __builtin_amdgcn_make_buffer_rsrc(a.x, 0, (int)ctl.blk_bytes, 0x00020000);
+  M.local = false;
+  const unsigned cvo = (unsigned)(ch.voff * (long)sizeof(V2));
+  auto boff = [&](const void *p) { return (unsigned)((const char *)p - M.base); };
+  const unsigned o_r[2] = {boff(a.r0) + cvo, boff(a.r1) + cvo};
+  const unsigned o_s[2] = {boff(a.s0) + cvo, boff(a.s1) + cvo};
+  const unsigned o_w[2] = {boff(a.w0) + cvo, boff(a.w1) + cvo};
+  const unsigned o_part[2] = {boff(ctl.part) + (unsigned)(16 * ch.pad),
+                              boff(ctl.part) + (unsigned)(16 * (ctl.nparts + ch.pad))};
+  const unsigned o_bar = boff(ctl.bar + 32 * tl.chain), o_bar0 = boff(ctl.bar0 + 32 * tl.chain);
+  const unsigned my_part = (unsigned)(16 * (tau - ch.tbeg));
 
   // links and r_0 = b on the region and its halo
   for (int e = tid; e < E2; e += NT) {
@@ -589,35 +671,45 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
               (unsigned)sizeof(V2);
     }
   }
-  V2 *const rb0 = a.r0 + ch.voff, *const rb1 = a.r1 + ch.voff;
-  V2 *const sb0 = a.s0 + ch.voff, *const sb1 = a.s1 + ch.voff;
-  V2 *const wb0 = a.w0 + ch.voff, *const wb1 = a.w1 + ch.voff;
+  // the XCD of every region of the chain (agent scope): local exchange when
+  // they are all on one
+  if (tid == 0)
+    __hip_atomic_store(ctl.xcc + tau, xcc_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!M.sync(o_bar0, (unsigned)ntc, ctl.err, red + 36, ctl.zero)) return;
+  if (tid == 0) {
+    const int mine = __hip_atomic_load(ctl.xcc + tau, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool same = true;
+    for (int k = ch.tbeg; k < ch.tend; k++)
+      same &= __hip_atomic_load(ctl.xcc + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == mine;
+    red[37] = same ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  M.local = red[37] != 0.0;
+
   double gam_old = 1.0, alpha_old = 1.0;
   int slot = 0;
-  // iteration -1 (set-up): w_0 = A b; T ta = 0 marks it
+  // iteration -1 (set-up): w_0 = A b
   bool setup = true;
   T ta = 0, tb = 0;
+  V2 hr[RES_NQ], hw[RES_NQ], hs[RES_NQ];  // halo values of the next iteration
   for (int j = -1;; j++) {
-    V2 *const rn_ = slot ? rb0 : rb1, *const sn_ = slot ? sb0 : sb1, *const wn_ = slot ? wb0 : wb1;
+    const int ns = setup ? 0 : slot ^ 1;  // parity written by this pass
     if (!setup) {
       // 3a. s_j, r_{j+1} on the halo and the region; p_j, x_{j+1}
-      const V2 *rc = slot ? rb1 : rb0, *wc = slot ? wb1 : wb0, *sc = slot ? sb1 : sb0;
 #pragma unroll
       for (int h = 0; h < RES_NQ; h++) {
         if (tid + h * NT >= RES_NR) continue;
-        const V2 hr = pub_load(at(rc, ri[h])), hw = pub_load(at(wc, ri[h]));
-        const V2 hs = pub_load(at(sc, ri[h]));
-        const V2 sn = cadd(hw, cscale(hs, tb));
-        R2[re[h]] = csub(hr, cscale(sn, ta));
+        const V2 sn = cadd(hw[h], cscale(hs[h], tb));
+        R2[re[h]] = csub(hr[h], cscale(sn, ta));
       }
 #pragma unroll
       for (int q = 0; q < NO; q++) {
         const int e = tid + q * NT, lt = e / TX, lx = e - lt * TX;
         const bool own = (inl >> q) & 1u;
         if (!own) {
-          r[q] = pub_load(at(rc, oi[q]));
-          w[q] = pub_load(at(wc, oi[q]));
-          s[q] = pub_load(at(sc, oi[q]));
+          r[q] = M.ld2(o_r[slot], oi[q]);
+          w[q] = M.ld2(o_w[slot], oi[q]);
+          s[q] = M.ld2(o_s[slot], oi[q]);
         }
         const V2 sn = cadd(w[q], cscale(s[q], tb));
         const V2 rn = csub(r[q], cscale(sn, ta));
@@ -629,16 +721,16 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
         r[q] = rn;
         s[q] = sn;
         if ((pub >> q) & 1u) {
-          pub_store(at(rn_, oi[q]), rn);
-          pub_store(at(sn_, oi[q]), sn);
+          M.st2(o_r[ns], oi[q], rn);
+          M.st2(o_s[ns], oi[q], sn);
         }
       }
     } else {
 #pragma unroll
       for (int q = 0; q < NO; q++)
         if ((pub >> q) & 1u) {
-          pub_store(at(rb0, oi[q]), r[q]);
-          pub_store(at(sb0, oi[q]), zero);
+          M.st2(o_r[0], oi[q], r[q]);
+          M.st2(o_s[0], oi[q], zero);
         }
     }
     __syncthreads();
@@ -674,7 +766,7 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       v = csub(v, cmul(kx, Y1[c1 + 1]));
       v = cadd(v, cjmul(kxd, Y1[c1 - 1]));
       w[q] = v;
-      if ((pub >> q) & 1u) pub_store(at(setup ? wb0 : wn_, oi[q]), v);
+      if ((pub >> q) & 1u) M.st2(o_w[ns], oi[q], v);
       gacc += (double)r[q].x * (double)r[q].x + (double)r[q].y * (double)r[q].y;
       dacc += (double)r[q].x * (double)v.x + (double)r[q].y * (double)v.y;
       __builtin_amdgcn_sched_barrier(0);  // one element's loads in flight at a time
@@ -687,27 +779,34 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       red[2 * (tid >> 6) + 1] = dacc;
     }
     __syncthreads();
-    const int nslot = setup ? 0 : slot ^ 1;
     if (tid == 0) {
       double g = 0.0, d = 0.0;
       for (int k = 0; k < NT / 64; k++) {
         g += red[2 * k];
         d += red[2 * k + 1];
       }
-      pub_store(a.part + ((long)nslot * a.ntiles + tau) * 2 + 0, g);
-      pub_store(a.part + ((long)nslot * a.ntiles + tau) * 2 + 1, d);
+      M.std_(o_part[ns], my_part, g);
+      M.std_(o_part[ns], my_part + 8, d);
     }
-    slot = nslot;
+    slot = ns;
     setup = false;
     // 1. every region of the chain has finished iteration j
-    if (!res_sync(bar, (unsigned)(j + 2) * (unsigned)ntc, ctl.err, red + 36)) break;
-    // 2. the chain's partials of iteration j+1 (same order in every region)
+    if (!M.sync(o_bar, (unsigned)(j + 2) * (unsigned)ntc, ctl.err, red + 36, ctl.zero)) break;
+    // 2. the halo layers of iteration j+1, issued before the partials so
+    // that the two memory latencies overlap
+#pragma unroll
+    for (int h = 0; h < RES_NQ; h++) {
+      if (tid + h * NT >= RES_NR) continue;
+      hr[h] = M.ld2(o_r[slot], ri[h]);
+      hw[h] = M.ld2(o_w[slot], ri[h]);
+      hs[h] = M.ld2(o_s[slot], ri[h]);
+    }
+    // the chain's partials of iteration j+1 (same order in every region)
     if (tid < 64) {
-      const double *pp = a.part + (long)slot * a.ntiles * 2;
       double pg = 0.0, pd = 0.0;
-      for (int k = ch.tbeg + tid; k < ch.tend; k += 64) {
-        pg += pub_load(pp + 2 * k);
-        pd += pub_load(pp + 2 * k + 1);
+      for (int k = tid; k < ntc; k += 64) {
+        pg += M.ldd(o_part[slot], 16u * k);
+        pd += M.ldd(o_part[slot], 16u * k + 8);
       }
       pg = wave_sum(pg);
       pd = wave_sum(pd);
@@ -914,6 +1013,7 @@ int build_batch(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T>
   ch.assign(nch, CgChain{});
   std::vector<CgTile> tiles;
   long voff = 0;
+  int pbeg = 0;
   for (int c = 0; c < nch; c++) {
     ch[c].G = G[c];
     ch[c].tbeg = (int)tiles.size();
@@ -926,7 +1026,8 @@ int build_batch(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T>
     ch[c].K = K[c];
     ch[c].stop_rr = stop_rr[c];
     ch[c].max_iter = max_iter[c];
-    ch[c].pad = 0;
+    ch[c].pad = pbeg;  // partial slot of the chain's first tile, on a 128-B boundary
+    pbeg += (ch[c].tend - ch[c].tbeg + 7) & ~7;
   }
   const int ntiles = (int)tiles.size();
   size_t vec = sizeof(c2<T>) * (size_t)voff;
@@ -1042,20 +1143,38 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
 {
   const int nreg = res_regions(nch, G);
   if (nreg < 1 || nreg > res_capacity(ctx)) return TH2D_ELAUNCH;
+  // control block: per chain an iteration counter and a set-up counter
+  // (a cache line each), the error flag, the XCD table and the partials
+  int nparts = 0;
+  for (int c = 0; c < nch; c++)
+    nparts += ((int)(((G[c].nt + RShape::TT - 1) / RShape::TT) *
+                     ((G[c].nx + RShape::TX - 1) / RShape::TX)) + 7) & ~7;
+  const size_t nzero = 64 * (size_t)nch + 32;  // counters and error flag, zeroed
+  const size_t nxcc = ((size_t)nreg + 31) & ~(size_t)31;
+  const size_t extra_bytes = 4 * (nzero + nxcc) + 32 * (size_t)nparts;
   CgBatch<float> a;
   std::vector<CgChain> ch;
   void *extra = nullptr;
   TH2D_TRY(build_batch<float>(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, RShape::TT,
-                              RShape::TX, sizeof(unsigned) * (nch + 1), a, ch, &extra));
+                              RShape::TX, extra_bytes, a, ch, &extra));
   if (a.ntiles != nreg) return TH2D_EARG;
   ResCtl ctl;
   ctl.bar = (unsigned *)extra;
-  ctl.err = (int *)(ctl.bar + nch);
+  ctl.bar0 = ctl.bar + 32 * nch;
+  ctl.err = (int *)(ctl.bar0 + 32 * nch);
+  ctl.xcc = ctl.err + 32;
+  ctl.part = (double *)(ctl.xcc + nxcc);
+  ctl.nparts = nparts;
+  const size_t blk = (size_t)((char *)extra + extra_bytes - (char *)a.x);
+  if (blk >= (size_t)1 << 31) return TH2D_ELAUNCH;  // buffer offsets are 32-bit
+  ctl.blk_bytes = (unsigned)blk;
+  ctl.zero = 0;
   ctx->cg_last_tile[0] = RShape::TT;
   ctx->cg_last_tile[1] = RShape::TX;
   ctx->cg_last_resident = 1;
   if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k0, ctx->stream));
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, ctx->stream, ctl.bar, nch + 1);
+  hipLaunchKernelGGL(k_zero_u32, dim3(std::min<int>(64, (int)(nzero + 255) / 256)), dim3(256), 0,
+                     ctx->stream, ctl.bar, (int)nzero);
   TH2D_HIP(hipGetLastError());
   void *args[] = {(void *)&a, (void *)&ctl};
   TH2D_HIP(hipLaunchCooperativeKernel(k_cg_resident<float>, dim3(nreg), dim3(RShape::NT), args, 0,
