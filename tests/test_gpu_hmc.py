@@ -305,6 +305,7 @@ def test_cg_resident_matches_step_kernel(hmc, oracle):
     batch = hmc.ChainBatch([s[0] for s in setups])
     ctx = setups[0][0].ctx
     runs = {}
+    N.lib().th2d_set_cg_reliable(ctx.ptr, 0)  # both kernels restart every pass
     try:
         for on in (1, 0):
             assert N.lib().th2d_set_cg_resident(ctx.ptr, on) == 0
@@ -314,6 +315,7 @@ def test_cg_resident_matches_step_kernel(hmc, oracle):
             runs[on] = (xs, infos)
     finally:
         N.lib().th2d_set_cg_resident(ctx.ptr, 1)
+        N.lib().th2d_set_cg_reliable(ctx.ptr, 0)
     for k, (lat, L, A, b, _) in enumerate(setups):
         for on in (1, 0):
             x, inf = runs[on][0][k], runs[on][1][k]
@@ -322,6 +324,35 @@ def test_cg_resident_matches_step_kernel(hmc, oracle):
             assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
         it1, it0 = runs[1][1][k].iterations, runs[0][1][k].iterations
         assert abs(it1 - it0) <= max(3, it0 // 10)
+
+
+@pytest.mark.parametrize("m", [0.2, 0.05])
+def test_cg_reliable_updates(hmc, oracle, m):
+    """Mixed solves with reliable updates (the FP32 Krylov sequence continues
+    across FP64 passes) and with restarts reach the same true residual; on a
+    well-conditioned system (m = 0.2) the updates save iterations (at
+    m = 0.05 they may cost some: the FP32 residual drifts within a pass)."""
+    from thirring2d_amd import _native as N
+    dims = [(128, 128), (64, 96), (96, 64)]
+    setups = [_setup(hmc, oracle, nt, nx, 600 + k, m=m) for k, (nt, nx) in enumerate(dims)]
+    batch = hmc.ChainBatch([s[0] for s in setups])
+    ctx = setups[0][0].ctx
+    its = {}
+    try:
+        for rel in (0, 1):
+            assert N.lib().th2d_set_cg_reliable(ctx.ptr, rel) == 0
+            xs, infos = batch.fmdm_invert_cg([s[3] for s in setups],
+                                             hmc.cg_params(tol_rel=1e-10, precision=N.PREC_MIXED))
+            assert _last_resident(ctx) == 1
+            for (lat, L, A, b, _), x, inf in zip(setups, xs, infos):
+                assert inf.status == 0
+                r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
+                assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
+            its[rel] = sum(inf.iterations for inf in infos)
+    finally:
+        N.lib().th2d_set_cg_reliable(ctx.ptr, 0)
+    if m >= 0.2:
+        assert its[1] <= its[0]
 
 
 def test_cg_resident_full_device(hmc, oracle):

@@ -68,6 +68,12 @@ struct CgChain {
   double stop_rr;
   int max_iter;
   int pad;          // resident solver: first partial slot of the chain (8-aligned)
+  // reliable update (resident solver): p, s of the previous pass in and
+  // out (chain vectors, FP32; null: none), and with cscale > 0 the solve
+  // continues that pass's Krylov sequence with p, s scaled by cscale and
+  // gam_old = gam0 cscale^2, alpha_old = alpha0
+  void *cp, *cs;
+  double cscale, gam0, alpha0;
 };
 
 struct CgTile {
@@ -83,6 +89,7 @@ struct CgResult {
   int iters, status;
   double rr;
   int done, pad;
+  double gam_old, alpha_old;  // resident solver: the last step's gam, alpha
 };
 
 template <typename T> struct CgBatch {
@@ -251,7 +258,7 @@ __global__ void __launch_bounds__(CG_MAX_THREADS) k_cg_init(CgBatch<T> a)
     write_tile_partial(a, c.red, 0, tau, gacc, dacc, v + (int)gridDim.x >= a.ntiles);
     if (tau == ch.tbeg && threadIdx.x == 0) {
       a.state[tl.chain] = CgState{1.0, 1.0, 0, 0};
-      a.res[tl.chain] = CgResult{0, TH2D_OK, 0.0, 0, 0};
+      a.res[tl.chain] = CgResult{0, TH2D_OK, 0.0, 0, 0, 0.0, 0.0};
     }
   }
 }
@@ -350,7 +357,7 @@ __global__ void __launch_bounds__(S::NT) k_cg_step(CgBatch<T> a, int slot)
       else stop = false;
     }
     if (stop) {
-      if (tau == ch.tbeg && tid == 0) a.res[tl.chain] = CgResult{j, status, gam, 1, 0};
+      if (tau == ch.tbeg && tid == 0) a.res[tl.chain] = CgResult{j, status, gam, 1, 0, 0.0, 0.0};
       __syncthreads();
       continue;
     }
@@ -433,27 +440,28 @@ __global__ void __launch_bounds__(S::NT) k_cg_step(CgBatch<T> a, int slot)
 //      y = M r_{j+1} on the 1-halo, w_{j+1} = M^dag y on the region;
 //   4. publishes its boundary layers of r_{j+1}, s_j, w_{j+1} and its
 //      partial of iteration j+1.
-// Regions are assigned to workgroups XCD by XCD, so a lattice of up to
-// ntiles/8 regions runs on one XCD; such a chain exchanges through that
-// XCD's L2, the others through the point of coherence of all XCDs (see
-// ResMem).  Every wave waits for its stores before thread 0 arrives.
-// A site is published when it lies within 2 sites of its
-// region's edge; every halo value that reaches an owned site's result is
-// such a site (a halo site two steps from a region lies within two steps
-// of the edge of the region it belongs to).  All regions of a chain take
-// bitwise the same stop decision and leave together; every wait is bounded
-// (a wait that times out sets the error flag and ends its workgroup).
+// The regions of a chain are placed on one XCD (place_regions), so the
+// exchange stays in that XCD's L2 (see ResMem); a chain larger than an XCD
+// is solved by the step kernel.  A site is published when it lies within 2
+// sites of its region's edge; every halo value that reaches an owned
+// site's result is such a site (a halo site two steps from a region lies
+// within two steps of the edge of the region it belongs to).  All regions
+// of a chain take bitwise the same stop decision and leave together; every
+// wait is bounded (a wait that times out sets the error flag and ends its
+// workgroup).
 using RShape = Shape<64, 64, 512>;
 constexpr int RES_NR = 4 * RShape::W2 + 4 * RShape::TT;  // halo elements of a region
 constexpr int RES_NQ = (RES_NR + RShape::NT - 1) / RShape::NT;
 constexpr unsigned RES_SPIN_LIMIT = 1u << 20;            // polls of one wait (about 1 s)
 
 struct ResCtl {
+  const int *wg_tile;  // [grid] region of each workgroup, -1: idle (see place_regions)
   unsigned *bar;     // [nchains][32] arrival counters of the iterations (own cache line each)
   unsigned *bar0;    // [nchains][32] set-up counters (agent scope)
   int *err;          // set by a wait that timed out
+  int *misplaced;    // set when a chain's regions do not share an XCD
   int *xcc;          // [ntiles] XCD of the workgroup of each region
-  double *part;      // [2][nparts][2] partials; a chain's start on a 128-B boundary
+  double *part;      // [2][nparts][4] partials; a chain's start on a 256-B boundary
   int nparts;        // partial slots per parity
   unsigned blk_bytes;  // bytes of the solver block from CgBatch::x (buffer range)
   unsigned zero;     // 0; an addend the compiler cannot see, so that a poll
@@ -498,43 +506,33 @@ __device__ __forceinline__ void l1_invalidate()
 }
 
 // Memory of the resident solver: the solver block at `base`, addressed by
-// a uniform byte offset plus a per-lane byte offset.  A chain whose
-// regions all run on one XCD exchanges through that XCD's L2 ("local"):
-// plain stores (the vector L1 writes through to L2), L2 atomics for the
-// arrival counter, polled with a returning read-modify-write (a plain or
+// a uniform byte offset plus a per-lane byte offset (buffer accesses).  All
+// regions of a chain run on one XCD and exchange through its L2: plain
+// stores (the vector L1 writes through to L2), arrival counters updated by
+// L2 atomics and polled with a returning read-modify-write (a plain or
 // scope-0 load of the counter may be served by a stale L1 line, and the
 // compiler turns an atomic add of a known 0 into such a load), then an L1
-// invalidation before the reads.  Other chains use agent-scope relaxed
-// atomics (written through to the point of coherence of all XCDs, read
-// from there).
+// invalidation before the reads (loads with cache policy sc0).
+typedef unsigned b64v __attribute__((ext_vector_type(2)));
 struct ResMem {
   char *base;
   __amdgpu_buffer_rsrc_t rs;
-  bool local;
 
   __device__ __forceinline__ float2 ld2(unsigned soff, unsigned voff) const
   {
-    if (local)
-      return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 1));
-    return pub_load((const float2 *)(base + soff + voff));
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 1));
   }
   __device__ __forceinline__ void st2(unsigned soff, unsigned voff, float2 v) const
   {
-    float2 *p = (float2 *)(base + soff + voff);
-    if (local) *p = v;
-    else pub_store(p, v);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(b64v, v), rs, voff, soff, 0);
   }
   __device__ __forceinline__ double ldd(unsigned soff, unsigned voff) const
   {
-    if (local)
-      return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 1));
-    return pub_load((const double *)(base + soff + voff));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 1));
   }
   __device__ __forceinline__ void std_(unsigned soff, unsigned voff, double v) const
   {
-    double *p = (double *)(base + soff + voff);
-    if (local) *p = v;
-    else pub_store(p, v);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(b64v, v), rs, voff, soff, 0);
   }
   // Arrive on the counter at byte offset `obar` and wait until it reaches
   // `target`; false when the wait timed out here or elsewhere (then the
@@ -545,14 +543,11 @@ struct ResMem {
     __syncthreads();
     if (threadIdx.x == 0) {
       unsigned *bar = (unsigned *)(base + obar);
-      if (local) __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      else __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       bool ok = true;
       for (unsigned n = 0;; n++) {
-        const unsigned v =
-            local ? __hip_atomic_fetch_add(bar, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-                  : __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (v >= target) break;
+        if (__hip_atomic_fetch_add(bar, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= target)
+          break;
         if (n >= RES_SPIN_LIMIT ||
             ((n & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
           __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -561,13 +556,41 @@ struct ResMem {
         }
         __builtin_amdgcn_s_sleep(1);
       }
-      if (local) l1_invalidate();  // one L1 per CU, and this workgroup is alone on it
+      l1_invalidate();  // one L1 per CU, and this workgroup is alone on it
       *flag = ok ? 1.0 : 0.0;
     }
     __syncthreads();
     return *flag != 0.0;
   }
 };
+
+// Agent-scope arrival counter for the set-up, before the regions of a chain
+// are known to share an XCD: the caller's stores have completed before it
+// arrives, and after the wait an acquire fence (L1 and L2 invalidation)
+// orders the reads that follow.
+__device__ bool agent_sync(unsigned *bar, unsigned target, int *err, double *flag)
+{
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores have completed
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool ok = true;
+    for (unsigned n = 0; __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target;
+         n++) {
+      if (n >= RES_SPIN_LIMIT ||
+          ((n & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = false;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    *flag = ok ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  return *flag != 0.0;
+}
 
 // Position (row, column) in the (TT+4) x (TX+4) halo frame of halo element
 // q: two rows above, two below, then two columns left and right.
@@ -608,8 +631,8 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   V2 *R2 = (V2 *)rlds, *Y1 = R2 + E2, *LKt = Y1 + E1, *LKx = LKt + E2;
   double *red = rlds + LDS_D;
   const int tid = threadIdx.x;
-  // regions XCD by XCD: a lattice of up to ntiles/8 regions runs on one XCD
-  const int tau = tile_of(blockIdx.x, a.ntiles);
+  const int tau = ctl.wg_tile[blockIdx.x];  // uniform: idle workgroups leave at once
+  if (tau < 0) return;
   const CgTile tl = a.tiles[tau];
   const CgChain ch = a.chains[tl.chain];
   const Geo G = ch.G;
@@ -621,16 +644,17 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   ResMem M;
   M.base = (char *)a.x;
   M.rs = __builtin_amdgcn_make_buffer_rsrc(a.x, 0, (int)ctl.blk_bytes, 0x00020000);
-  M.local = false;
   const unsigned cvo = (unsigned)(ch.voff * (long)sizeof(V2));
   auto boff = [&](const void *p) { return (unsigned)((const char *)p - M.base); };
   const unsigned o_r[2] = {boff(a.r0) + cvo, boff(a.r1) + cvo};
   const unsigned o_s[2] = {boff(a.s0) + cvo, boff(a.s1) + cvo};
   const unsigned o_w[2] = {boff(a.w0) + cvo, boff(a.w1) + cvo};
-  const unsigned o_part[2] = {boff(ctl.part) + (unsigned)(16 * ch.pad),
-                              boff(ctl.part) + (unsigned)(16 * (ctl.nparts + ch.pad))};
-  const unsigned o_bar = boff(ctl.bar + 32 * tl.chain), o_bar0 = boff(ctl.bar0 + 32 * tl.chain);
-  const unsigned my_part = (unsigned)(16 * (tau - ch.tbeg));
+  // partial slots of 4 doubles: (r.r, r.w) each iteration, and in the
+  // set-up of a continued pass also (r.s, p.s) of the carried p, s
+  const unsigned o_part[2] = {boff(ctl.part) + (unsigned)(32 * ch.pad),
+                              boff(ctl.part) + (unsigned)(32 * (ctl.nparts + ch.pad))};
+  const unsigned o_bar = boff(ctl.bar + 32 * tl.chain);
+  const unsigned my_part = (unsigned)(32 * (tau - ch.tbeg));
 
   // links and r_0 = b on the region and its halo
   for (int e = tid; e < E2; e += NT) {
@@ -657,6 +681,11 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
     }
     r[q] = *at(b, oi[q]);
     w[q] = s[q] = p[q] = x[q] = zero;
+    if (ch.cscale > 0.0 && lt < tte && lx < txe) {  // continue the previous pass
+      const T cf = (T)ch.cscale;
+      p[q] = cscale(*at((const V2 *)ch.cp, oi[q]), cf);
+      s[q] = cscale(*at((const V2 *)ch.cs, oi[q]), cf);
+    }
   }
   unsigned ri[RES_NQ];  // byte offsets of the halo elements
   int re[RES_NQ];
@@ -671,22 +700,27 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
               (unsigned)sizeof(V2);
     }
   }
-  // the XCD of every region of the chain (agent scope): local exchange when
-  // they are all on one
+  // the regions of the chain were placed on one XCD (workgroups are
+  // dispatched round-robin over the XCDs); check it with the XCD register
+  // (agent scope), and leave without solving if not (the host then solves
+  // the batch with the step kernel)
   if (tid == 0)
     __hip_atomic_store(ctl.xcc + tau, xcc_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (!M.sync(o_bar0, (unsigned)ntc, ctl.err, red + 36, ctl.zero)) return;
+  if (!agent_sync(ctl.bar0 + 32 * tl.chain, (unsigned)ntc, ctl.err, red + 36)) return;
   if (tid == 0) {
     const int mine = __hip_atomic_load(ctl.xcc + tau, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     bool same = true;
     for (int k = ch.tbeg; k < ch.tend; k++)
       same &= __hip_atomic_load(ctl.xcc + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == mine;
     red[37] = same ? 1.0 : 0.0;
+    if (!same) __hip_atomic_store(ctl.misplaced, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   __syncthreads();
-  M.local = red[37] != 0.0;
+  if (red[37] == 0.0) return;  // the same decision in every region of the chain
 
-  double gam_old = 1.0, alpha_old = 1.0;
+  const bool cont = ch.cscale > 0.0;
+  double gam_old = cont ? ch.gam0 * ch.cscale * ch.cscale : 1.0;
+  double alpha_old = cont ? ch.alpha0 : 1.0;
   int slot = 0;
   // iteration -1 (set-up): w_0 = A b
   bool setup = true;
@@ -730,7 +764,7 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       for (int q = 0; q < NO; q++)
         if ((pub >> q) & 1u) {
           M.st2(o_r[0], oi[q], r[q]);
-          M.st2(o_s[0], oi[q], zero);
+          M.st2(o_s[0], oi[q], s[q]);
         }
     }
     __syncthreads();
@@ -751,8 +785,9 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       Y1[e] = y;
     }
     __syncthreads();
-    // 3c. w_{j+1} = M^dag y on the region; partials of iteration j+1
-    double gacc = 0.0, dacc = 0.0;
+    // 3c. w_{j+1} = M^dag y on the region; partials of iteration j+1 (a
+    // thread's sites in FP32, then FP64)
+    T gacc32 = 0, dacc32 = 0, racc32 = 0, sacc32 = 0;
 #pragma unroll
     for (int q = 0; q < NO; q++) {
       if (!((inl >> q) & 1u)) continue;
@@ -767,26 +802,35 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       v = cadd(v, cjmul(kxd, Y1[c1 - 1]));
       w[q] = v;
       if ((pub >> q) & 1u) M.st2(o_w[ns], oi[q], v);
-      gacc += (double)r[q].x * (double)r[q].x + (double)r[q].y * (double)r[q].y;
-      dacc += (double)r[q].x * (double)v.x + (double)r[q].y * (double)v.y;
-      __builtin_amdgcn_sched_barrier(0);  // one element's loads in flight at a time
+      gacc32 += r[q].x * r[q].x + r[q].y * r[q].y;
+      dacc32 += r[q].x * v.x + r[q].y * v.y;
+      if (setup && cont) {
+        racc32 += r[q].x * s[q].x + r[q].y * s[q].y;
+        sacc32 += p[q].x * s[q].x + p[q].y * s[q].y;
+      }
     }
     // 4. the region's partial (fixed order: lanes, then waves)
-    gacc = wave_sum(gacc);
-    dacc = wave_sum(dacc);
-    if ((tid & 63) == 0) {
-      red[2 * (tid >> 6)] = gacc;
-      red[2 * (tid >> 6) + 1] = dacc;
+    const int npart = setup && cont ? 4 : 2;
+    {
+      const double g = wave_sum((double)gacc32), d = wave_sum((double)dacc32);
+      double rs = 0.0, ps = 0.0;
+      if (npart == 4) {
+        rs = wave_sum((double)racc32);
+        ps = wave_sum((double)sacc32);
+      }
+      if ((tid & 63) == 0) {
+        red[4 * (tid >> 6)] = g;
+        red[4 * (tid >> 6) + 1] = d;
+        red[4 * (tid >> 6) + 2] = rs;
+        red[4 * (tid >> 6) + 3] = ps;
+      }
     }
     __syncthreads();
     if (tid == 0) {
-      double g = 0.0, d = 0.0;
-      for (int k = 0; k < NT / 64; k++) {
-        g += red[2 * k];
-        d += red[2 * k + 1];
-      }
-      M.std_(o_part[ns], my_part, g);
-      M.std_(o_part[ns], my_part + 8, d);
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int k = 0; k < NT / 64; k++)
+        for (int u = 0; u < 4; u++) acc[u] += red[4 * k + u];
+      for (int u = 0; u < npart; u++) M.std_(o_part[ns], my_part + 8 * u, acc[u]);
     }
     slot = ns;
     setup = false;
@@ -801,48 +845,62 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       hw[h] = M.ld2(o_w[slot], ri[h]);
       hs[h] = M.ld2(o_s[slot], ri[h]);
     }
-    // the chain's partials of iteration j+1 (same order in every region)
-    if (tid < 64) {
-      double pg = 0.0, pd = 0.0;
-      for (int k = tid; k < ntc; k += 64) {
-        pg += M.ldd(o_part[slot], 16u * k);
-        pd += M.ldd(o_part[slot], 16u * k + 8);
-      }
-      pg = wave_sum(pg);
-      pd = wave_sum(pd);
-      if (tid == 0) {
-        red[32] = pg;
-        red[33] = pd;
+    // the chain's partials of iteration j+1, summed by every wave in the
+    // same order (a butterfly leaves the total in every lane), so that all
+    // waves of all regions of the chain hold bitwise the same gam, del
+    const int jn = j + 1;
+    const bool first_cont = jn == 0 && cont;
+    double gam = 0.0, del = 0.0, rho = 0.0, sig = 0.0;
+    for (int k = tid & 63; k < ntc; k += 64) {
+      gam += M.ldd(o_part[slot], 32u * k);
+      del += M.ldd(o_part[slot], 32u * k + 8);
+      if (first_cont) {
+        rho += M.ldd(o_part[slot], 32u * k + 16);
+        sig += M.ldd(o_part[slot], 32u * k + 24);
       }
     }
-    __syncthreads();
-    const double gam = red[32], del = red[33];
-    const int jn = j + 1;
+    gam = wave_sum(gam);
+    del = wave_sum(del);
+    if (first_cont) {
+      rho = wave_sum(rho);
+      sig = wave_sum(sig);
+    }
     int status = 1;  // 1: continue
     double alpha = 0.0, beta = 0.0;
     if (gam < ch.stop_rr) status = TH2D_OK;
     else if (jn >= ch.max_iter) status = TH2D_ENOCONV;
     else if (!(del > 0.0) || !isfinite(gam)) status = TH2D_EBREAKDOWN;
     else {
-      beta = jn == 0 ? 0.0 : gam / gam_old;
-      alpha = jn == 0 ? gam / del : gam / (del - beta * gam / alpha_old);
+      const bool fresh = jn == 0 && !cont;
+      beta = fresh ? 0.0 : gam / gam_old;
+      // p.Ap from the CG relations, except after a residual replacement,
+      // which breaks them: there p.Ap = r.w + 2 beta r.s + beta^2 p.s
+      alpha = fresh ? gam / del
+                    : gam / (first_cont ? del + 2.0 * beta * rho + beta * beta * sig
+                                        : del - beta * gam / alpha_old);
       if (!(alpha > 0.0) || !isfinite(alpha)) status = TH2D_EBREAKDOWN;
     }
     if (status != 1) {
-      if (first && tid == 0) a.res[tl.chain] = CgResult{jn, status, gam, 1, 0};
+      if (first && tid == 0)
+        a.res[tl.chain] = CgResult{jn, status, gam, 1, 0, gam_old, alpha_old};
       break;
     }
     gam_old = gam;
     alpha_old = alpha;
     ta = (T)alpha;
     tb = (T)beta;
-    // red[32..33] are rewritten only after the next two barriers
   }
-  // x of the owned sites
+  // x of the owned sites, and p, s for a reliable update
   V2 *xo = a.x + ch.voff;
 #pragma unroll
   for (int q = 0; q < NO; q++)
-    if ((inl >> q) & 1u) *at(xo, oi[q]) = x[q];
+    if ((inl >> q) & 1u) {
+      *at(xo, oi[q]) = x[q];
+      if (ch.cp) {
+        *at((V2 *)ch.cp, oi[q]) = p[q];
+        *at((V2 *)ch.cs, oi[q]) = s[q];
+      }
+    }
 }
 
 // Default tile shape for a batch: among the compiled shapes no larger
@@ -921,6 +979,14 @@ struct InnerResult {
   int iters;
   double rr;
   int status;
+  double gam, alpha;  // resident solver: state for a reliable update
+};
+
+// Carried Krylov state of a chain between passes of the mixed solver (a
+// reliable update): p, s in and out; cscale > 0 continues.
+struct Carry {
+  void *p, *s;
+  double cscale, gam0, alpha0;
 };
 
 // Batched inner solve: A_c y_c = rhs_c from y_c = 0, stop r.r < stop_c.
@@ -929,7 +995,7 @@ template <typename T>
 int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
                 const c2<T> *const *rhs,
                 const c2<T> *const *K, const double *stop_rr, const int *max_iter, c2<T> **xout,
-                InnerResult *res, int force_shape = -1);
+                InnerResult *res, int force_shape = -1, const Carry *carry = nullptr);
 
 // Key of a batch for the autotuner: the distinct lattice shapes and the
 // order of magnitude of the site count (a restart pass over a subset of
@@ -1008,7 +1074,9 @@ int autotune_shape(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2
 template <typename T>
 int build_batch(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T> *const *rhs,
                 const c2<T> *const *K, const double *stop_rr, const int *max_iter, int tt, int tx,
-                size_t extra, CgBatch<T> &a, std::vector<CgChain> &ch, void **extra_ptr)
+                size_t extra, CgBatch<T> &a, std::vector<CgChain> &ch, void **extra_ptr,
+                const Carry *carry = nullptr, const void *extra_init = nullptr,
+                size_t extra_init_bytes = 0)
 {
   ch.assign(nch, CgChain{});
   std::vector<CgTile> tiles;
@@ -1026,8 +1094,13 @@ int build_batch(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T>
     ch[c].K = K[c];
     ch[c].stop_rr = stop_rr[c];
     ch[c].max_iter = max_iter[c];
-    ch[c].pad = pbeg;  // partial slot of the chain's first tile, on a 128-B boundary
+    ch[c].pad = pbeg;  // partial slot of the chain's first tile (8-aligned: a cache-line boundary)
     pbeg += (ch[c].tend - ch[c].tbeg + 7) & ~7;
+    ch[c].cp = carry ? carry[c].p : nullptr;
+    ch[c].cs = carry ? carry[c].s : nullptr;
+    ch[c].cscale = carry ? carry[c].cscale : 0.0;
+    ch[c].gam0 = carry ? carry[c].gam0 : 0.0;
+    ch[c].alpha0 = carry ? carry[c].alpha0 : 0.0;
   }
   const int ntiles = (int)tiles.size();
   size_t vec = sizeof(c2<T>) * (size_t)voff;
@@ -1066,12 +1139,16 @@ int build_batch(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T>
   a.nchains = nch;
   a.TT = tt;
   a.TX = tx;
-  // tables through a pinned staging block
-  size_t hbytes = std::max(chb + tlb, rsb);
+  // tables (and the first bytes of `extra`, which follow them) through a
+  // pinned staging block, in one copy
+  if (extra_init_bytes > exb) return TH2D_EARG;
+  size_t hbytes = std::max(chb + tlb + extra_init_bytes, rsb);
   TH2D_TRY(th2d_host_reserve(ctx->redh, hbytes + 64));
   memcpy(ctx->redh.ptr, ch.data(), sizeof(CgChain) * nch);
   memcpy((char *)ctx->redh.ptr + chb, tiles.data(), sizeof(CgTile) * ntiles);
-  TH2D_HIP(hipMemcpyAsync(dch, ctx->redh.ptr, chb + tlb, hipMemcpyHostToDevice, ctx->stream));
+  if (extra_init_bytes) memcpy((char *)ctx->redh.ptr + chb + tlb, extra_init, extra_init_bytes);
+  TH2D_HIP(hipMemcpyAsync(dch, ctx->redh.ptr, chb + tlb + extra_init_bytes, hipMemcpyHostToDevice,
+                          ctx->stream));
   return TH2D_OK;
 }
 
@@ -1102,6 +1179,8 @@ int finish_solve(th2d_ctx *ctx, int nch, const Geo *G, const CgBatch<T> &a,
     res[c].iters = h[c].iters;
     res[c].rr = h[c].rr;
     res[c].status = h[c].status;
+    res[c].gam = h[c].gam_old;
+    res[c].alpha = h[c].alpha_old;
     xout[c] = a.x + ch[c].voff;
   }
   return TH2D_OK;
@@ -1137,60 +1216,107 @@ int res_capacity(th2d_ctx *ctx)
 // One inner solve as a single cooperative launch of the resident kernel
 // (FP32); TH2D_ELAUNCH when the batch has more regions than the device
 // holds resident workgroups.
+// Places the regions of every chain on one XCD.  Workgroups are dispatched
+// round-robin over the XCDs, so workgroups x, x + 8, x + 16, ... run on
+// XCD x; chains are packed largest first onto the least-loaded XCD with
+// room for them (at most per_xcd regions each).  Fills wg_tile (the tile of
+// each workgroup, -1: idle) and returns the grid, or 0 when a chain does
+// not fit on one XCD.  The kernel checks the placement with the XCD
+// register.
+constexpr int kNumXcd = 8;
+int place_regions(int nch, const Geo *G, int per_xcd, std::vector<int> &wg_tile)
+{
+  std::vector<int> ntc(nch), tbeg(nch + 1, 0), order(nch);
+  for (int c = 0; c < nch; c++) {
+    ntc[c] = ((G[c].nt + RShape::TT - 1) / RShape::TT) * ((G[c].nx + RShape::TX - 1) / RShape::TX);
+    tbeg[c + 1] = tbeg[c] + ntc[c];
+    order[c] = c;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return ntc[p] > ntc[q]; });
+  std::vector<int> load(kNumXcd, 0);
+  std::vector<std::vector<int>> on(kNumXcd);
+  for (int c : order) {
+    int best = -1;
+    for (int x = 0; x < kNumXcd; x++)
+      if (load[x] + ntc[c] <= per_xcd && (best < 0 || load[x] < load[best])) best = x;
+    if (best < 0) return 0;
+    on[best].push_back(c);
+    load[best] += ntc[c];
+  }
+  const int slots = *std::max_element(load.begin(), load.end());
+  wg_tile.assign((size_t)kNumXcd * slots, -1);
+  for (int x = 0; x < kNumXcd; x++) {
+    int slot = 0;
+    for (int c : on[x])
+      for (int t = tbeg[c]; t < tbeg[c + 1]; t++) wg_tile[(size_t)slot++ * kNumXcd + x] = t;
+  }
+  return kNumXcd * slots;
+}
+
 int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
                    const c2<float> *const *rhs, const c2<float> *const *K, const double *stop_rr,
-                   const int *max_iter, c2<float> **xout, InnerResult *res)
+                   const int *max_iter, c2<float> **xout, InnerResult *res, const Carry *carry)
 {
   const int nreg = res_regions(nch, G);
-  if (nreg < 1 || nreg > res_capacity(ctx)) return TH2D_ELAUNCH;
-  // control block: per chain an iteration counter and a set-up counter
-  // (a cache line each), the error flag, the XCD table and the partials
+  const int cap = res_capacity(ctx);
+  if (nreg < 1 || nreg > cap) return TH2D_ELAUNCH;
+  std::vector<int> wg_tile;
+  const int grid = place_regions(nch, G, cap / kNumXcd, wg_tile);
+  if (grid < 1 || grid > cap) return TH2D_ELAUNCH;
+  // control block: the workgroup table, per chain an iteration counter and a
+  // set-up counter (a cache line each), the error flags, the XCD table and
+  // the partials
   int nparts = 0;
   for (int c = 0; c < nch; c++)
     nparts += ((int)(((G[c].nt + RShape::TT - 1) / RShape::TT) *
                      ((G[c].nx + RShape::TX - 1) / RShape::TX)) + 7) & ~7;
-  const size_t nzero = 64 * (size_t)nch + 32;  // counters and error flag, zeroed
+  const size_t ntab = ((size_t)grid + 31) & ~(size_t)31;
+  const size_t nzero = 64 * (size_t)nch + 32;  // counters and error flags, zeroed
   const size_t nxcc = ((size_t)nreg + 31) & ~(size_t)31;
-  const size_t extra_bytes = 4 * (nzero + nxcc) + 32 * (size_t)nparts;
+  const size_t extra_bytes = 4 * (ntab + nzero + nxcc) + 64 * (size_t)nparts;  // 2 x 4 doubles
   CgBatch<float> a;
   std::vector<CgChain> ch;
   void *extra = nullptr;
   TH2D_TRY(build_batch<float>(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, RShape::TT,
-                              RShape::TX, extra_bytes, a, ch, &extra));
+                              RShape::TX, extra_bytes, a, ch, &extra, carry, wg_tile.data(),
+                              sizeof(int) * wg_tile.size()));
   if (a.ntiles != nreg) return TH2D_EARG;
   ResCtl ctl;
-  ctl.bar = (unsigned *)extra;
+  ctl.wg_tile = (const int *)extra;
+  ctl.bar = (unsigned *)extra + ntab;
   ctl.bar0 = ctl.bar + 32 * nch;
   ctl.err = (int *)(ctl.bar0 + 32 * nch);
+  ctl.misplaced = ctl.err + 1;
   ctl.xcc = ctl.err + 32;
-  ctl.part = (double *)(ctl.xcc + nxcc);
+  ctl.part = (double *)(ctl.xcc + nxcc);  // 128-B aligned: every count above is a multiple of 32
   ctl.nparts = nparts;
   const size_t blk = (size_t)((char *)extra + extra_bytes - (char *)a.x);
   if (blk >= (size_t)1 << 31) return TH2D_ELAUNCH;  // buffer offsets are 32-bit
   ctl.blk_bytes = (unsigned)blk;
   ctl.zero = 0;
-  ctx->cg_last_tile[0] = RShape::TT;
-  ctx->cg_last_tile[1] = RShape::TX;
-  ctx->cg_last_resident = 1;
   if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k0, ctx->stream));
   hipLaunchKernelGGL(k_zero_u32, dim3(std::min<int>(64, (int)(nzero + 255) / 256)), dim3(256), 0,
                      ctx->stream, ctl.bar, (int)nzero);
   TH2D_HIP(hipGetLastError());
   void *args[] = {(void *)&a, (void *)&ctl};
-  TH2D_HIP(hipLaunchCooperativeKernel(k_cg_resident<float>, dim3(nreg), dim3(RShape::NT), args, 0,
+  TH2D_HIP(hipLaunchCooperativeKernel(k_cg_resident<float>, dim3(grid), dim3(RShape::NT), args, 0,
                                       ctx->stream));
-  // results and the error flag in one copy
+  // results and the two flags in one copy
   const size_t rb = sizeof(CgResult) * nch;
   TH2D_TRY(th2d_host_reserve(ctx->redh, rb + 64));
   TH2D_HIP(hipMemcpyAsync(ctx->redh.ptr, a.res, rb, hipMemcpyDeviceToHost, ctx->stream));
-  TH2D_HIP(hipMemcpyAsync((char *)ctx->redh.ptr + rb, ctl.err, sizeof(int), hipMemcpyDeviceToHost,
-                          ctx->stream));
+  TH2D_HIP(hipMemcpyAsync((char *)ctx->redh.ptr + rb, ctl.err, 2 * sizeof(int),
+                          hipMemcpyDeviceToHost, ctx->stream));
   TH2D_HIP(hipStreamSynchronize(ctx->stream));
   std::vector<CgResult> h(nch);
   memcpy(h.data(), ctx->redh.ptr, rb);
-  int err = 0;
-  memcpy(&err, (char *)ctx->redh.ptr + rb, sizeof(int));
-  if (err) return TH2D_ESYNC;
+  int flags[2] = {0, 0};
+  memcpy(flags, (char *)ctx->redh.ptr + rb, sizeof(flags));
+  if (flags[1]) return TH2D_ELAUNCH;  // placement not as dispatched: the step kernel solves
+  if (flags[0]) return TH2D_ESYNC;
+  ctx->cg_last_tile[0] = RShape::TT;
+  ctx->cg_last_tile[1] = RShape::TX;
+  ctx->cg_last_resident = 1;
   return finish_solve<float>(ctx, nch, G, a, ch, h, 1, xout, res);
 }
 
@@ -1198,11 +1324,11 @@ template <typename T>
 int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
                 const c2<T> *const *rhs,
                 const c2<T> *const *K, const double *stop_rr, const int *max_iter, c2<T> **xout,
-                InnerResult *res, int force_shape)
+                InnerResult *res, int force_shape, const Carry *carry)
 {
   if constexpr (sizeof(T) == 4) {
     if (force_shape < 0 && ctx->cg_resident && !(ctx->tile_t > 0 && ctx->tile_x > 0)) {
-      const int rc = resident_solve(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, xout, res);
+      const int rc = resident_solve(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, xout, res, carry);
       if (rc != TH2D_ELAUNCH) return rc;
     }
   }
@@ -1298,6 +1424,13 @@ extern "C" int th2d_set_cg_autotune(th2d_ctx *ctx, int on)
   return TH2D_OK;
 }
 
+extern "C" int th2d_set_cg_reliable(th2d_ctx *ctx, int on)
+{
+  if (!ctx) return TH2D_EARG;
+  ctx->cg_reliable = on ? 1 : 0;
+  return TH2D_OK;
+}
+
 extern "C" int th2d_set_cg_resident(th2d_ctx *ctx, int on)
 {
   if (!ctx) return TH2D_EARG;
@@ -1360,8 +1493,12 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
   double *R = (double *)ctx->misc.ptr;
   double *TMP = R + 2 * Vt;
   double *dsc = TMP + 2 * Vt;
-  TH2D_TRY(th2d_buf_reserve(ctx->cgmix, sizeof(float) * 2 * Vt + 256));
+  // FP32 right-hand sides, and p, s carried between passes (reliable updates)
+  TH2D_TRY(th2d_buf_reserve(ctx->cgmix, sizeof(float) * 2 * 3 * Vt + 256));
   float2 *R32 = (float2 *)ctx->cgmix.ptr;
+  float2 *P32 = R32 + Vt, *S32 = P32 + Vt;
+  std::vector<char> carry_ok(nch, 0);
+  std::vector<double> cgam(nch, 0.0), calpha(nch, 0.0), nprev(nch, 0.0);
 
   std::vector<th2d_cg_info> inf(nch);
   std::vector<double> bb(nch), rr(nch), target(nch);
@@ -1450,15 +1587,35 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
     // inner solve, then x += |r| y for every chain and the true residual
     // r = b - M^dag M x in FP64: five batched launches
     std::vector<BOp> tx(na), tm(na), tmd(na), tsub(na), tdot(na);
+    std::vector<char> was_cont(na, 0);
     if (mixed) {
       std::vector<const float2 *> rh(na), kk(na);
       std::vector<float2 *> xo(na);
+      // reliable update: a chain whose previous pass left its p, s continues
+      // that Krylov sequence with r replaced by the true residual (rescaled
+      // from the previous pass's normalisation to this one's)
+      std::vector<Carry> cy(na);
       for (int k = 0; k < na; k++) {
-        rh[k] = R32 + voff[idx[k]];
-        kk[k] = (const float2 *)K32[idx[k]];
+        const int c = idx[k];
+        rh[k] = R32 + voff[c];
+        kk[k] = (const float2 *)K32[c];
+        cy[k] = Carry{P32 + voff[c], S32 + voff[c], 0.0, 0.0, 0.0};
+        if (outer > 0 && carry_ok[c] && ctx->cg_reliable && cgam[c] > 0.0 && calpha[c] > 0.0) {
+          cy[k].cscale = nprev[c] / sqrt(rr[c]);
+          cy[k].gam0 = cgam[c];
+          cy[k].alpha0 = calpha[c];
+          was_cont[k] = 1;
+        }
+        nprev[c] = sqrt(rr[c]);
       }
       TH2D_TRY(inner_solve<float>(ctx, ctx->cg32, outer, na, g2.data(), rh.data(), kk.data(),
-                                  stop.data(), mi.data(), xo.data(), ir.data()));
+                                  stop.data(), mi.data(), xo.data(), ir.data(), -1, cy.data()));
+      for (int k = 0; k < na; k++) {  // (kept only if the pass stays consistent, below)
+        const int c = idx[k];
+        carry_ok[c] = ctx->cg_last_resident && ir[k].status != TH2D_EBREAKDOWN;
+        cgam[c] = ir[k].gam;
+        calpha[c] = ir[k].alpha;
+      }
       for (int k = 0; k < na; k++) {
         tx[k] = op_of(idx[k]);
         tx[k].d0 = x[idx[k]];
@@ -1524,9 +1681,19 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
       if (outer > 0) inf[c].restarts++;
       double rnew = rn[k];
       bool breakdown = ir[k].status == TH2D_EBREAKDOWN;
+      // the carried Krylov state is consistent with the true residual only
+      // when the FP32 recursive residual has not drifted from it:
+      // |r_true|^2 <= 1.1 |r_rec|^2 (drift below a third of the residual);
+      // otherwise the next pass restarts
+      if (mixed && !(rnew <= 1.1 * ir[k].rr * rr[c])) carry_ok[c] = 0;
       if (abs_mode && !mixed && ir[k].status == TH2D_OK) {  // reference stop met
         rr[c] = rnew;
         active[c] = 0;
+        continue;
+      }
+      if (breakdown && was_cont[k]) {  // the continuation broke down: restart afresh
+        rr[c] = std::min(rr[c], rnew);
+        if (rr[c] <= target[c]) active[c] = 0;
         continue;
       }
       if (breakdown && !(rnew < rr[c])) {

@@ -61,6 +61,7 @@ struct th2d_ctx {
   int cg_last_tile[2] = {0, 0};   // tile shape of the last inner solve
   int cg_autotune = 0;            // measure tile shapes per batch signature
   int cg_resident = 1;            // FP32 inner solves in the resident kernel when they fit
+  int cg_reliable = 0;            // mixed solves continue the Krylov sequence across passes
   int res_capacity = -1;          // resident workgroups the device holds (-1: not queried)
   int cg_last_resident = 0;       // the last inner solve ran resident
   std::map<unsigned long long, int> cg_tuned[2];  // signature -> shape (FP32, FP64)
