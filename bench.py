@@ -12,7 +12,9 @@ a batch of independent 256^2 chains advanced in lockstep (every inversion
 of all chains is one batched solve) with mixed precision (FP32 CG, FP64
 true-residual restarts); the batched FP64 path and single chains in FP64
 and mixed precision are timed beside it.  `roofline` is for the dominant
-kernel, the CG step kernel, timed with HIP events on its stream.
+kernel, timed with HIP events on its stream: the resident FP32 solver (one
+cooperative launch per inner solve) when the batch fits the device's
+resident workgroups, as 16 chains of 256^2 do, else the CG step kernel.
 """
 from __future__ import annotations
 
@@ -28,8 +30,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# measured HBM bytes per k_cg_step launch (profiles/k_cg_step_hmc256_32chains_mixed.md)
-TRAFFIC_BYTES = {(256, 32): 199.6e6}
+# measured HBM bytes per launch of the dominant kernel (FETCH_SIZE + WRITE_SIZE,
+# copy-calibrated): k_cg_resident at 16 chains
+# (profiles/k_cg_resident_hmc256_16chains_mixed.md), k_cg_step at 32 chains
+# (profiles/k_cg_step_hmc256_32chains_mixed.md)
+# key: (lattice size, chains, resident solver)
+TRAFFIC_BYTES = {(256, 32, False): 199.6e6}
 
 
 def parse():
@@ -38,7 +44,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--chains", type=int, default=32)
+    ap.add_argument("--chains", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dslash", action="store_true")
     ap.add_argument("--only-headline", action="store_true",
@@ -288,10 +294,14 @@ def main():
     # headline: batched mixed precision
     dt_mx, res_mx, kms, kn, kbytes = run_hmc(H, N, size, N.PREC_MIXED, args.steps, args.warmup,
                                              chains=nch)
+    import ctypes as C
+    on = C.c_int(0)
+    N.check(N.lib().th2d_cg_last_resident(H.Context.get().ptr, C.byref(on)), "cg_last_resident")
+    resident = bool(on.value)
     if args.only_headline:
         print(json.dumps({"metric": "batched mixed HMC (profiling run)", "value": nch * args.steps / dt_mx,
                           "unit": "traj/s", "chains": nch, "kernel_avg_ms": kms / max(kn, 1),
-                          "launches": kn}), flush=True)
+                          "launches": kn, "resident": resident}), flush=True)
         return
     small = max(2, args.steps // 4)
     dt_b64, res_b64, *_ = run_hmc(H, N, size, N.PREC_F64, small, 1, chains=nch)
@@ -333,9 +343,17 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      # HBM bytes per launch of this kernel at this workload, from the
                      # FETCH_SIZE / WRITE_SIZE passes in profiles/ (copy-calibrated)
-                     "traffic": TRAFFIC_BYTES.get((size, nch)),
-                     "kernel": "k_cg_step<float> (CG iteration, 120 B/site algorithmic)",
-                     "launches": kn, "avg_launch_ms": kms / max(kn, 1)},
+                     "traffic": TRAFFIC_BYTES.get((size, nch, resident)),
+                     "kernel": ("k_cg_resident<float> (one launch per FP32 inner solve; "
+                                "CG iteration 120 B/site algorithmic)" if resident else
+                                "k_cg_step<float> (CG iteration, 120 B/site algorithmic)"),
+                     "launches": kn, "avg_launch_ms": kms / max(kn, 1),
+                     "note": ("the resident solver keeps each region's vectors in registers "
+                              "and its links in LDS for the whole solve, so its HBM traffic "
+                              "(`traffic`) is far below the streaming CG's algorithmic bytes "
+                              "and frac can exceed 1; it is bound by the per-iteration "
+                              "synchronisation of each chain's regions" if resident else
+                              "streaming: every iteration reads and writes the vectors")},
     }
     if not args.no_dslash:
         d = {}

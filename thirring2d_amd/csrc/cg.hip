@@ -24,6 +24,7 @@
 // host enqueues launches in chunks and reads the per-chain stop flags once
 // per chunk; tiles of a stopped chain return at once.
 #include "th2d_internal.h"
+#include "res_layout.h"
 
 #include <algorithm>
 #include <cmath>
@@ -445,14 +446,21 @@ __global__ void __launch_bounds__(S::NT) k_cg_step(CgBatch<T> a, int slot)
 // is solved by the step kernel.  A site is published when it lies within 2
 // sites of its region's edge; every halo value that reaches an owned
 // site's result is such a site (a halo site two steps from a region lies
-// within two steps of the edge of the region it belongs to).  All regions
+// within two steps of the edge of the region it belongs to).  Sites are
+// published into a compact block per region (res_layout.h), not at their
+// places in the lattice vectors: neighbours then read whole rows and column
+// pairs from consecutive addresses, and the exchange of the 32 regions an
+// XCD holds (24 KB each) stays in its L2.  Through the lattice vectors a
+// column layer costs a 128-B line per site and the exchange missed L2.  All regions
 // of a chain take bitwise the same stop decision and leave together; every
 // wait is bounded (a wait that times out sets the error flag and ends its
 // workgroup).
 using RShape = Shape<64, 64, 512>;
+static_assert(RShape::TT == RES_T && RShape::TX == RES_T, "regions of the publication layout");
 constexpr int RES_NR = 4 * RShape::W2 + 4 * RShape::TT;  // halo elements of a region
 constexpr int RES_NQ = (RES_NR + RShape::NT - 1) / RShape::NT;
 constexpr unsigned RES_SPIN_LIMIT = 1u << 20;            // polls of one wait (about 1 s)
+constexpr unsigned kNoLoad = 0xFFFFFFFFu;                 // halo element no result needs
 
 struct ResCtl {
   const int *wg_tile;  // [grid] region of each workgroup, -1: idle (see place_regions)
@@ -462,6 +470,8 @@ struct ResCtl {
   int *misplaced;    // set when a chain's regions do not share an XCD
   int *xcc;          // [ntiles] XCD of the workgroup of each region
   double *part;      // [2][nparts][4] partials; a chain's start on a 256-B boundary
+  void *pub;         // [2][3][ntiles][RES_PUB] published boundary layers of r, s, w
+                     // per parity (res_layout.h)
   int nparts;        // partial slots per parity
   unsigned blk_bytes;  // bytes of the solver block from CgBatch::x (buffer range)
   unsigned zero;     // 0; an addend the compiler cannot see, so that a poll
@@ -644,11 +654,29 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   ResMem M;
   M.base = (char *)a.x;
   M.rs = __builtin_amdgcn_make_buffer_rsrc(a.x, 0, (int)ctl.blk_bytes, 0x00020000);
-  const unsigned cvo = (unsigned)(ch.voff * (long)sizeof(V2));
   auto boff = [&](const void *p) { return (unsigned)((const char *)p - M.base); };
-  const unsigned o_r[2] = {boff(a.r0) + cvo, boff(a.r1) + cvo};
-  const unsigned o_s[2] = {boff(a.s0) + cvo, boff(a.s1) + cvo};
-  const unsigned o_w[2] = {boff(a.w0) + cvo, boff(a.w1) + cvo};
+  // publication planes of r, s, w per parity, and this region's block
+  constexpr unsigned PB = RES_PUB * sizeof(V2);
+  const unsigned plane = (unsigned)a.ntiles * PB, o_pub = boff(ctl.pub);
+  const unsigned o_r[2] = {o_pub, o_pub + 3 * plane};
+  const unsigned o_s[2] = {o_pub + plane, o_pub + 4 * plane};
+  const unsigned o_w[2] = {o_pub + 2 * plane, o_pub + 5 * plane};
+  const unsigned my_blk = (unsigned)tau * PB;
+  // publication offset of owned element (lt, lx) within 2 sites of the edge
+  auto pub_at = [&](int lt, int lx) {
+    return my_blk + (unsigned)res_pub_slot(lt, lx, tte, txe) * (unsigned)sizeof(V2);
+  };
+  // publication offset of the site at region coordinates (lt, lx) outside
+  // the owned block, or kNoLoad when no owned result depends on it
+  auto src_at = [&](int lt, int lx) {
+    unsigned off = kNoLoad;
+    if (lt < tte + 2 && lx < txe + 2) {
+      int k;
+      const int sl = res_site_slot(wrapn(tl.t0 + lt, G.nt), wrapn(tl.x0 + lx, G.nx), G.nt, G.nx, k);
+      if (sl >= 0) off = (unsigned)(ch.tbeg + k) * PB + (unsigned)sl * (unsigned)sizeof(V2);
+    }
+    return off;
+  };
   // partial slots of 4 doubles: (r.r, r.w) each iteration, and in the
   // set-up of a continued pass also (r.s, p.s) of the carried p, s
   const unsigned o_part[2] = {boff(ctl.part) + (unsigned)(32 * ch.pad),
@@ -666,38 +694,39 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   }
   // owned elements e = tid + q NT of the region; those outside the lattice
   // (a region larger than what is left of it) are halo, reloaded each time
-  unsigned oi[NO];  // byte offsets of the owned elements in a chain vector
+  // (byte offset of element q in a chain vector)
+  auto oi_of = [&](int q) {
+    const int e = tid + q * NT, lt = e / TX, lx = e - lt * TX;
+    return (unsigned)eo_index(G, wrapn(tl.t0 + lt, G.nt), wrapn(tl.x0 + lx, G.nx)) *
+           (unsigned)sizeof(V2);
+  };
   unsigned inl = 0, pub = 0;
   V2 r[NO], w[NO], s[NO], p[NO], x[NO];
   const V2 zero = cmk<V2>((T)0, (T)0);
 #pragma unroll
   for (int q = 0; q < NO; q++) {
     const int e = tid + q * NT, lt = e / TX, lx = e - lt * TX;
-    oi[q] = (unsigned)eo_index(G, wrapn(tl.t0 + lt, G.nt), wrapn(tl.x0 + lx, G.nx)) *
-            (unsigned)sizeof(V2);
+    const unsigned oi = oi_of(q);
     if (lt < tte && lx < txe) {
       inl |= 1u << q;
       if (lt < 2 || lt >= tte - 2 || lx < 2 || lx >= txe - 2) pub |= 1u << q;
     }
-    r[q] = *at(b, oi[q]);
+    r[q] = *at(b, oi);
     w[q] = s[q] = p[q] = x[q] = zero;
     if (ch.cscale > 0.0 && lt < tte && lx < txe) {  // continue the previous pass
       const T cf = (T)ch.cscale;
-      p[q] = cscale(*at((const V2 *)ch.cp, oi[q]), cf);
-      s[q] = cscale(*at((const V2 *)ch.cs, oi[q]), cf);
+      p[q] = cscale(*at((const V2 *)ch.cp, oi), cf);
+      s[q] = cscale(*at((const V2 *)ch.cs, oi), cf);
     }
   }
-  unsigned ri[RES_NQ];  // byte offsets of the halo elements
-  int re[RES_NQ];
+  unsigned ri[RES_NQ];  // publication offsets of the halo elements (kNoLoad: none)
 #pragma unroll
   for (int h = 0; h < RES_NQ; h++) {
-    ri[h] = re[h] = 0;
+    ri[h] = kNoLoad;
     if (tid + h * NT < RES_NR) {
       int et, ex;
       res_ring_pos(tid + h * NT, et, ex);
-      re[h] = et * W2 + ex;
-      ri[h] = (unsigned)eo_index(G, wrapn(tl.t0 - 2 + et, G.nt), wrapn(tl.x0 - 2 + ex, G.nx)) *
-              (unsigned)sizeof(V2);
+      ri[h] = src_at(et - 2, ex - 2);
     }
   }
   // the regions of the chain were placed on one XCD (workgroups are
@@ -734,16 +763,22 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       for (int h = 0; h < RES_NQ; h++) {
         if (tid + h * NT >= RES_NR) continue;
         const V2 sn = cadd(hw[h], cscale(hs[h], tb));
-        R2[re[h]] = csub(hr[h], cscale(sn, ta));
+        int et, ex;
+        res_ring_pos(tid + h * NT, et, ex);
+        R2[et * W2 + ex] = csub(hr[h], cscale(sn, ta));
       }
 #pragma unroll
       for (int q = 0; q < NO; q++) {
         const int e = tid + q * NT, lt = e / TX, lx = e - lt * TX;
         const bool own = (inl >> q) & 1u;
-        if (!own) {
-          r[q] = M.ld2(o_r[slot], oi[q]);
-          w[q] = M.ld2(o_w[slot], oi[q]);
-          s[q] = M.ld2(o_s[slot], oi[q]);
+        if (!own) {  // outside the lattice part of the region: the owner's values
+          const unsigned so = src_at(lt, lx);
+          r[q] = w[q] = s[q] = zero;
+          if (so != kNoLoad) {
+            r[q] = M.ld2(o_r[slot], so);
+            w[q] = M.ld2(o_w[slot], so);
+            s[q] = M.ld2(o_s[slot], so);
+          }
         }
         const V2 sn = cadd(w[q], cscale(s[q], tb));
         const V2 rn = csub(r[q], cscale(sn, ta));
@@ -755,16 +790,19 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
         r[q] = rn;
         s[q] = sn;
         if ((pub >> q) & 1u) {
-          M.st2(o_r[ns], oi[q], rn);
-          M.st2(o_s[ns], oi[q], sn);
+          const unsigned po = pub_at(lt, lx);
+          M.st2(o_r[ns], po, rn);
+          M.st2(o_s[ns], po, sn);
         }
       }
     } else {
 #pragma unroll
       for (int q = 0; q < NO; q++)
         if ((pub >> q) & 1u) {
-          M.st2(o_r[0], oi[q], r[q]);
-          M.st2(o_s[0], oi[q], s[q]);
+          const int e = tid + q * NT, lt = e / TX, lx = e - lt * TX;
+          const unsigned po = pub_at(lt, lx);
+          M.st2(o_r[0], po, r[q]);
+          M.st2(o_s[0], po, s[q]);
         }
     }
     __syncthreads();
@@ -801,7 +839,7 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       v = csub(v, cmul(kx, Y1[c1 + 1]));
       v = cadd(v, cjmul(kxd, Y1[c1 - 1]));
       w[q] = v;
-      if ((pub >> q) & 1u) M.st2(o_w[ns], oi[q], v);
+      if ((pub >> q) & 1u) M.st2(o_w[ns], pub_at(lt, lx), v);
       gacc32 += r[q].x * r[q].x + r[q].y * r[q].y;
       dacc32 += r[q].x * v.x + r[q].y * v.y;
       if (setup && cont) {
@@ -826,28 +864,33 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       }
     }
     __syncthreads();
-    if (tid == 0) {
+    if (tid < 64) {  // the waves' sums in lanes 0..NT/64-1 of wave 0, fixed butterfly order
+      constexpr int NW = NT / 64;
+      static_assert((NW & (NW - 1)) == 0 && NW <= 64, "waves per workgroup");
       double acc[4] = {0.0, 0.0, 0.0, 0.0};
-      for (int k = 0; k < NT / 64; k++)
-        for (int u = 0; u < 4; u++) acc[u] += red[4 * k + u];
-      for (int u = 0; u < npart; u++) M.std_(o_part[ns], my_part + 8 * u, acc[u]);
+      if (tid < NW)
+        for (int u = 0; u < 4; u++) acc[u] = red[4 * tid + u];
+#pragma unroll
+      for (int off = 1; off < NW; off <<= 1) {
+        acc[0] += __shfl_xor(acc[0], off, 64);
+        acc[1] += __shfl_xor(acc[1], off, 64);
+        if (npart == 4) {
+          acc[2] += __shfl_xor(acc[2], off, 64);
+          acc[3] += __shfl_xor(acc[3], off, 64);
+        }
+      }
+      if (tid == 0)
+        for (int u = 0; u < npart; u++) M.std_(o_part[ns], my_part + 8 * u, acc[u]);
     }
     slot = ns;
     setup = false;
     // 1. every region of the chain has finished iteration j
     if (!M.sync(o_bar, (unsigned)(j + 2) * (unsigned)ntc, ctl.err, red + 36, ctl.zero)) break;
-    // 2. the halo layers of iteration j+1, issued before the partials so
-    // that the two memory latencies overlap
-#pragma unroll
-    for (int h = 0; h < RES_NQ; h++) {
-      if (tid + h * NT >= RES_NR) continue;
-      hr[h] = M.ld2(o_r[slot], ri[h]);
-      hw[h] = M.ld2(o_w[slot], ri[h]);
-      hs[h] = M.ld2(o_s[slot], ri[h]);
-    }
-    // the chain's partials of iteration j+1, summed by every wave in the
+    // 2. the chain's partials of iteration j+1, summed by every wave in the
     // same order (a butterfly leaves the total in every lane), so that all
-    // waves of all regions of the chain hold bitwise the same gam, del
+    // waves of all regions of the chain hold bitwise the same gam, del; then
+    // the halo layers of iteration j+1 (used from step 3a on: the sums only
+    // wait for the partials)
     const int jn = j + 1;
     const bool first_cont = jn == 0 && cont;
     double gam = 0.0, del = 0.0, rho = 0.0, sig = 0.0;
@@ -857,6 +900,16 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       if (first_cont) {
         rho += M.ldd(o_part[slot], 32u * k + 16);
         sig += M.ldd(o_part[slot], 32u * k + 24);
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < RES_NQ; h++) {
+      if (tid + h * NT >= RES_NR) continue;
+      hr[h] = hw[h] = hs[h] = zero;
+      if (ri[h] != kNoLoad) {
+        hr[h] = M.ld2(o_r[slot], ri[h]);
+        hw[h] = M.ld2(o_w[slot], ri[h]);
+        hs[h] = M.ld2(o_s[slot], ri[h]);
       }
     }
     gam = wave_sum(gam);
@@ -895,10 +948,11 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
 #pragma unroll
   for (int q = 0; q < NO; q++)
     if ((inl >> q) & 1u) {
-      *at(xo, oi[q]) = x[q];
+      const unsigned oi = oi_of(q);
+      *at(xo, oi) = x[q];
       if (ch.cp) {
-        *at((V2 *)ch.cp, oi[q]) = p[q];
-        *at((V2 *)ch.cs, oi[q]) = s[q];
+        *at((V2 *)ch.cp, oi) = p[q];
+        *at((V2 *)ch.cs, oi) = s[q];
       }
     }
 }
@@ -1264,8 +1318,8 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   const int grid = place_regions(nch, G, cap / kNumXcd, wg_tile);
   if (grid < 1 || grid > cap) return TH2D_ELAUNCH;
   // control block: the workgroup table, per chain an iteration counter and a
-  // set-up counter (a cache line each), the error flags, the XCD table and
-  // the partials
+  // set-up counter (a cache line each), the error flags, the XCD table, the
+  // partials and the publication planes
   int nparts = 0;
   for (int c = 0; c < nch; c++)
     nparts += ((int)(((G[c].nt + RShape::TT - 1) / RShape::TT) *
@@ -1273,7 +1327,8 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   const size_t ntab = ((size_t)grid + 31) & ~(size_t)31;
   const size_t nzero = 64 * (size_t)nch + 32;  // counters and error flags, zeroed
   const size_t nxcc = ((size_t)nreg + 31) & ~(size_t)31;
-  const size_t extra_bytes = 4 * (ntab + nzero + nxcc) + 64 * (size_t)nparts;  // 2 x 4 doubles
+  const size_t pub_bytes = 6 * (size_t)nreg * RES_PUB * sizeof(c2<float>);  // 2 x (r, s, w)
+  const size_t extra_bytes = 4 * (ntab + nzero + nxcc) + 64 * (size_t)nparts + pub_bytes;
   CgBatch<float> a;
   std::vector<CgChain> ch;
   void *extra = nullptr;
@@ -1289,6 +1344,7 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   ctl.misplaced = ctl.err + 1;
   ctl.xcc = ctl.err + 32;
   ctl.part = (double *)(ctl.xcc + nxcc);  // 128-B aligned: every count above is a multiple of 32
+  ctl.pub = (char *)ctl.part + 64 * (size_t)nparts;  // 512-B aligned: nparts is a multiple of 8
   ctl.nparts = nparts;
   const size_t blk = (size_t)((char *)extra + extra_bytes - (char *)a.x);
   if (blk >= (size_t)1 << 31) return TH2D_ELAUNCH;  // buffer offsets are 32-bit
