@@ -292,15 +292,23 @@ def _last_resident(ctx):
     return on.value
 
 
-def test_cg_resident_matches_step_kernel(hmc, oracle):
+@pytest.mark.parametrize("mu1", [0.0, 0.25])
+def test_cg_resident_matches_step_kernel(hmc, oracle, mu1):
     """The resident FP32 solver (one cooperative launch, vectors in
     registers, boundary layers exchanged through memory) and the step kernel
-    solve the same mixed-precision batch: regions cut by the lattice edge,
-    lattices smaller than a region and several regions per lattice; both
-    reach the true residual, with iteration counts within 10%."""
+    solve the same mixed-precision batch: lattices smaller than a region
+    (one clipped region that exchanges with itself) and lattices of several
+    full regions; both reach the true residual, with iteration counts within
+    10%.  With
+    mu1 != 0 one chain has a chemical potential (the kernel instance with
+    the e^{+-mu} factors) and a larger mass; M is then not normal, and the
+    FP32 passes of the two kernels round differently enough that only the
+    true residual is compared for that chain."""
     from thirring2d_amd import _native as N
-    dims = [(128, 96), (64, 64), (40, 72), (16, 16), (96, 200)]
-    setups = [_setup(hmc, oracle, nt, nx, 300 + k, m=0.05 + 0.03 * k)
+    dims = [(128, 128), (64, 64), (40, 48), (16, 16), (64, 192), (8, 60)]
+    setups = [_setup(hmc, oracle, nt, nx, 300 + k,
+                     m=(0.3 if k == 1 and mu1 else 0.05 + 0.03 * k),
+                     mu=mu1 if k == 1 else 0.0)
               for k, (nt, nx) in enumerate(dims)]
     batch = hmc.ChainBatch([s[0] for s in setups])
     ctx = setups[0][0].ctx
@@ -322,6 +330,8 @@ def test_cg_resident_matches_step_kernel(hmc, oracle):
             assert inf.status == 0
             r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
             assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
+        if k == 1 and mu1:
+            continue
         it1, it0 = runs[1][1][k].iterations, runs[0][1][k].iterations
         assert abs(it1 - it0) <= max(3, it0 // 10)
 
@@ -333,7 +343,7 @@ def test_cg_reliable_updates(hmc, oracle, m):
     well-conditioned system (m = 0.2) the updates save iterations (at
     m = 0.05 they may cost some: the FP32 residual drifts within a pass)."""
     from thirring2d_amd import _native as N
-    dims = [(128, 128), (64, 96), (96, 64)]
+    dims = [(128, 128), (64, 128), (128, 64)]
     setups = [_setup(hmc, oracle, nt, nx, 600 + k, m=m) for k, (nt, nx) in enumerate(dims)]
     batch = hmc.ChainBatch([s[0] for s in setups])
     ctx = setups[0][0].ctx
@@ -374,9 +384,59 @@ def test_cg_resident_full_device(hmc, oracle):
         assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
 
 
+def test_cg_resident_groups(hmc, oracle):
+    """A batch with more regions than the device holds resident workgroups
+    (12 x 256^2 + 20 x 128^2 = 272 regions) is solved by the resident kernel
+    in two launches, one per group of chains; every chain reaches the true
+    residual, with the step kernel's iteration counts within 10%."""
+    from thirring2d_amd import _native as N
+    dims = [(256, 256)] * 12 + [(128, 128)] * 20
+    setups = [_setup(hmc, oracle, nt, nx, 700 + k, m=0.1 + 0.002 * k)
+              for k, (nt, nx) in enumerate(dims)]
+    batch = hmc.ChainBatch([s[0] for s in setups])
+    ctx = setups[0][0].ctx
+    runs = {}
+    N.lib().th2d_set_cg_reliable(ctx.ptr, 0)
+    try:
+        for on in (1, 0):
+            assert N.lib().th2d_set_cg_resident(ctx.ptr, on) == 0
+            xs, infos = batch.fmdm_invert_cg([s[3] for s in setups],
+                                             hmc.cg_params(tol_rel=1e-10, precision=N.PREC_MIXED))
+            assert _last_resident(ctx) == on
+            runs[on] = (xs, infos)
+    finally:
+        N.lib().th2d_set_cg_resident(ctx.ptr, 1)
+    for k, (lat, L, A, b, _) in enumerate(setups):
+        inf = runs[1][1][k]
+        assert inf.status == 0 and inf.true_rel_resid <= 1e-10
+        if k % 7 == 0:  # the oracle's residual on a sample of the chains
+            x = runs[1][0][k]
+            r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
+            assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
+        it1, it0 = inf.iterations, runs[0][1][k].iterations
+        assert abs(it1 - it0) <= max(3, it0 // 10)
+
+
+def test_cg_resident_declines_clipped_regions(hmc, oracle):
+    """A batch with a lattice cut into several regions with a clipped edge
+    region (96^2: regions of 64 and 32 sites a side) is solved by the step
+    kernel; the solution reaches the true residual."""
+    from thirring2d_amd import _native as N
+    setups = [_setup(hmc, oracle, nt, nx, 600 + k, m=0.1)
+              for k, (nt, nx) in enumerate([(96, 96), (64, 64)])]
+    batch = hmc.ChainBatch([s[0] for s in setups])
+    xs, infos = batch.fmdm_invert_cg([s[3] for s in setups],
+                                     hmc.cg_params(tol_rel=1e-10, precision=N.PREC_MIXED))
+    assert _last_resident(setups[0][0].ctx) == 0
+    for (lat, L, A, b, _), x, inf in zip(setups, xs, infos):
+        assert inf.status == 0
+        r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
+        assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
+
+
 def test_cg_resident_falls_back_when_too_large(hmc, oracle):
-    """A batch with more regions than resident workgroups (5 x 512^2 = 320
-    regions) runs in the step kernel."""
+    """A batch with a lattice larger than an XCD's resident workgroups
+    (512^2 = 64 regions) runs in the step kernel."""
     from thirring2d_amd import _native as N
     setups = [_setup(hmc, oracle, 512, 512, 500 + k, m=0.2) for k in range(5)]
     batch = hmc.ChainBatch([s[0] for s in setups])

@@ -628,7 +628,8 @@ __device__ __forceinline__ int xcc_id()
   return (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u);
 }
 
-template <typename T>
+// MU0: every chain of the batch has mu = 0 (the e^{+-mu} factors are 1)
+template <typename T, bool MU0>
 __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
 {
   using S = RShape;
@@ -650,7 +651,7 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   const bool first = tau == ch.tbeg;
   const int tte = min(TT, G.nt - tl.t0), txe = min(TX, G.nx - tl.x0);
   const V2 *K = (const V2 *)ch.K, *b = (const V2 *)ch.b;
-  const T m = (T)G.m, epm = (T)G.ep, emm = (T)G.em;
+  const T m = (T)G.m, epm = MU0 ? (T)1 : (T)G.ep, emm = MU0 ? (T)1 : (T)G.em;
   ResMem M;
   M.base = (char *)a.x;
   M.rs = __builtin_amdgcn_make_buffer_rsrc(a.x, 0, (int)ctl.blk_bytes, 0x00020000);
@@ -758,15 +759,8 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   for (int j = -1;; j++) {
     const int ns = setup ? 0 : slot ^ 1;  // parity written by this pass
     if (!setup) {
-      // 3a. s_j, r_{j+1} on the halo and the region; p_j, x_{j+1}
-#pragma unroll
-      for (int h = 0; h < RES_NQ; h++) {
-        if (tid + h * NT >= RES_NR) continue;
-        const V2 sn = cadd(hw[h], cscale(hs[h], tb));
-        int et, ex;
-        res_ring_pos(tid + h * NT, et, ex);
-        R2[et * W2 + ex] = csub(hr[h], cscale(sn, ta));
-      }
+      // 3a. s_j, r_{j+1} on the region and the halo; p_j, x_{j+1} (the
+      // region first: it does not wait for the halo loads)
 #pragma unroll
       for (int q = 0; q < NO; q++) {
         const int e = tid + q * NT, lt = e / TX, lx = e - lt * TX;
@@ -795,6 +789,14 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
           M.st2(o_s[ns], po, sn);
         }
       }
+#pragma unroll
+      for (int h = 0; h < RES_NQ; h++) {
+        if (tid + h * NT >= RES_NR) continue;
+        const V2 sn = cadd(hw[h], cscale(hs[h], tb));
+        int et, ex;
+        res_ring_pos(tid + h * NT, et, ex);
+        R2[et * W2 + ex] = csub(hr[h], cscale(sn, ta));
+      }
     } else {
 #pragma unroll
       for (int q = 0; q < NO; q++)
@@ -815,11 +817,15 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       const int ft = e / W1, fx = e - ft * W1;
       const int c = (ft + 1) * W2 + (fx + 1);
       const V2 kt = LKt[c], kx = LKx[c], ktd = LKt[c - W2], kxd = LKx[c - 1];
-      V2 y = cscale(R2[c], m);
-      y = cadd(y, cscale(cmul(kt, R2[c + W2]), epm));
-      y = csub(y, cscale(cjmul(ktd, R2[c - W2]), emm));
-      y = cadd(y, cmul(kx, R2[c + 1]));
-      y = csub(y, cjmul(kxd, R2[c - 1]));
+      const V2 r0 = R2[c], rtp = R2[c + W2], rtm = R2[c - W2], rxp = R2[c + 1], rxm = R2[c - 1];
+      // all nine loads in flight before the arithmetic: one LDS latency per
+      // element (two waves per SIMD hide little)
+      __builtin_amdgcn_sched_barrier(0);
+      V2 y = cscale(r0, m);
+      y = cadd(y, cscale(cmul(kt, rtp), epm));
+      y = csub(y, cscale(cjmul(ktd, rtm), emm));
+      y = cadd(y, cmul(kx, rxp));
+      y = csub(y, cjmul(kxd, rxm));
       Y1[e] = y;
     }
     __syncthreads();
@@ -833,11 +839,13 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       const int c1 = (lt + 1) * W1 + (lx + 1);
       const int c2i = (lt + 2) * W2 + (lx + 2);
       const V2 kt = LKt[c2i], kx = LKx[c2i], ktd = LKt[c2i - W2], kxd = LKx[c2i - 1];
-      V2 v = cscale(Y1[c1], m);
-      v = csub(v, cscale(cmul(kt, Y1[c1 + W1]), emm));
-      v = cadd(v, cscale(cjmul(ktd, Y1[c1 - W1]), epm));
-      v = csub(v, cmul(kx, Y1[c1 + 1]));
-      v = cadd(v, cjmul(kxd, Y1[c1 - 1]));
+      const V2 y0 = Y1[c1], ytp = Y1[c1 + W1], ytm = Y1[c1 - W1], yxp = Y1[c1 + 1], yxm = Y1[c1 - 1];
+      __builtin_amdgcn_sched_barrier(0);  // one LDS latency per element (as in 3b)
+      V2 v = cscale(y0, m);
+      v = csub(v, cscale(cmul(kt, ytp), emm));
+      v = cadd(v, cscale(cjmul(ktd, ytm), epm));
+      v = csub(v, cmul(kx, yxp));
+      v = cadd(v, cjmul(kxd, yxm));
       w[q] = v;
       if ((pub >> q) & 1u) M.st2(o_w[ns], pub_at(lt, lx), v);
       gacc32 += r[q].x * r[q].x + r[q].y * r[q].y;
@@ -1254,12 +1262,24 @@ int res_regions(int nch, const Geo *G)
   return n > (1L << 30) ? (1 << 30) : (int)n;
 }
 
+// Lattices the resident solver takes: one region (both sides at most 64),
+// or sides that are multiples of 64.  A lattice cut into several regions
+// with a clipped edge region broke down now and then on the GPU when the
+// XCDs were full (wrong halo values in those chains only), while a CPU
+// emulation of the kernel with random schedules solved the same batches
+// exactly; until the cause is known the step kernel solves such batches.
+bool res_eligible(const Geo &g)
+{
+  const bool one = g.nt <= RES_T && g.nx <= RES_T;
+  return one || (g.nt % RES_T == 0 && g.nx % RES_T == 0);
+}
+
 // Resident workgroups the device holds (cooperative launch bound).
 int res_capacity(th2d_ctx *ctx)
 {
   if (ctx->res_capacity < 0) {
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)k_cg_resident<float>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)k_cg_resident<float, true>,
                                                      RShape::NT, 0) != hipSuccess)
       occ = 0;
     ctx->res_capacity = occ * ctx->num_cu;
@@ -1267,56 +1287,96 @@ int res_capacity(th2d_ctx *ctx)
   return ctx->res_capacity;
 }
 
-// One inner solve as a single cooperative launch of the resident kernel
-// (FP32); TH2D_ELAUNCH when the batch has more regions than the device
-// holds resident workgroups.
 // Places the regions of every chain on one XCD.  Workgroups are dispatched
-// round-robin over the XCDs, so workgroups x, x + 8, x + 16, ... run on
-// XCD x; chains are packed largest first onto the least-loaded XCD with
-// room for them (at most per_xcd regions each).  Fills wg_tile (the tile of
-// each workgroup, -1: idle) and returns the grid, or 0 when a chain does
-// not fit on one XCD.  The kernel checks the placement with the XCD
-// register.
+// round-robin over the XCDs, so workgroups x, x + 8, x + 16, ... of a
+// launch run on XCD x.  Chains are packed largest first: into the first
+// group (one cooperative launch) with an XCD that has room for the whole
+// chain (at most per_xcd regions per XCD), onto its least-loaded such XCD;
+// a new group when none has.  Fills wg_tile with every group's table (the
+// tile of each workgroup, -1: idle) and `groups` with each group's grid and
+// table offset; false when a chain does not fit on one XCD.  The kernel
+// checks the placement with the XCD register.
 constexpr int kNumXcd = 8;
-int place_regions(int nch, const Geo *G, int per_xcd, std::vector<int> &wg_tile)
+struct ResGroup {
+  int grid;        // workgroups of the launch (a multiple of 8)
+  size_t tab;      // offset of its table in wg_tile
+  long sites;      // lattice sites it solves
+};
+bool place_regions(int nch, const Geo *G, int per_xcd, std::vector<int> &wg_tile,
+                   std::vector<ResGroup> &groups)
 {
   std::vector<int> ntc(nch), tbeg(nch + 1, 0), order(nch);
   for (int c = 0; c < nch; c++) {
     ntc[c] = ((G[c].nt + RShape::TT - 1) / RShape::TT) * ((G[c].nx + RShape::TX - 1) / RShape::TX);
+    if (ntc[c] > per_xcd) return false;
     tbeg[c + 1] = tbeg[c] + ntc[c];
     order[c] = c;
   }
   std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return ntc[p] > ntc[q]; });
-  std::vector<int> load(kNumXcd, 0);
-  std::vector<std::vector<int>> on(kNumXcd);
+  std::vector<std::vector<int>> load;                // [group][xcd]
+  std::vector<std::vector<std::vector<int>>> on;     // [group][xcd] chains
+  std::vector<long> sites;
   for (int c : order) {
-    int best = -1;
-    for (int x = 0; x < kNumXcd; x++)
-      if (load[x] + ntc[c] <= per_xcd && (best < 0 || load[x] < load[best])) best = x;
-    if (best < 0) return 0;
-    on[best].push_back(c);
-    load[best] += ntc[c];
+    int gb = -1, best = -1;
+    for (int g = 0; g < (int)load.size() && gb < 0; g++)
+      for (int x = 0; x < kNumXcd; x++)
+        if (load[g][x] + ntc[c] <= per_xcd && (best < 0 || load[g][x] < load[g][best])) {
+          gb = g;
+          best = x;
+        }
+    if (gb < 0) {
+      gb = (int)load.size();
+      best = 0;
+      load.emplace_back(kNumXcd, 0);
+      on.emplace_back(kNumXcd);
+      sites.push_back(0);
+    }
+    on[gb][best].push_back(c);
+    load[gb][best] += ntc[c];
+    sites[gb] += 2L * G[c].vh;
   }
-  const int slots = *std::max_element(load.begin(), load.end());
-  wg_tile.assign((size_t)kNumXcd * slots, -1);
-  for (int x = 0; x < kNumXcd; x++) {
-    int slot = 0;
-    for (int c : on[x])
-      for (int t = tbeg[c]; t < tbeg[c + 1]; t++) wg_tile[(size_t)slot++ * kNumXcd + x] = t;
+  wg_tile.clear();
+  groups.clear();
+  for (size_t g = 0; g < load.size(); g++) {
+    const int slots = *std::max_element(load[g].begin(), load[g].end());
+    const size_t tab = wg_tile.size();
+    wg_tile.resize(tab + (size_t)kNumXcd * slots, -1);
+    for (int x = 0; x < kNumXcd; x++) {
+      int slot = 0;
+      for (int c : on[g][x])
+        for (int t = tbeg[c]; t < tbeg[c + 1]; t++) wg_tile[tab + (size_t)slot++ * kNumXcd + x] = t;
+    }
+    groups.push_back(ResGroup{kNumXcd * slots, tab, sites[g]});
   }
-  return kNumXcd * slots;
+  return true;
 }
 
+// One inner solve as cooperative launches of the resident kernel (FP32),
+// one per group of chains that fits the device's resident workgroups;
+// TH2D_ELAUNCH when a chain does not fit on one XCD, or when the groups are
+// modelled slower than the step kernel: an iteration of a group costs
+// about 8.5 us (measured at 16 lattices of 256^2, set by the per-iteration
+// synchronisation), one of the step kernel about 10 us plus 120 B per site
+// at 4.5 TB/s (measured).  Many small lattices favour the step kernel.
 int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
                    const c2<float> *const *rhs, const c2<float> *const *K, const double *stop_rr,
                    const int *max_iter, c2<float> **xout, InnerResult *res, const Carry *carry)
 {
   const int nreg = res_regions(nch, G);
   const int cap = res_capacity(ctx);
-  if (nreg < 1 || nreg > cap) return TH2D_ELAUNCH;
+  if (nreg < 1 || cap < kNumXcd) return TH2D_ELAUNCH;
+  for (int c = 0; c < nch; c++)
+    if (!res_eligible(G[c])) return TH2D_ELAUNCH;
   std::vector<int> wg_tile;
-  const int grid = place_regions(nch, G, cap / kNumXcd, wg_tile);
-  if (grid < 1 || grid > cap) return TH2D_ELAUNCH;
+  std::vector<ResGroup> groups;
+  if (!place_regions(nch, G, cap / kNumXcd, wg_tile, groups)) return TH2D_ELAUNCH;
+  long sites = 0;
+  for (const ResGroup &g : groups) {
+    if (g.grid < 1 || g.grid > cap) return TH2D_ELAUNCH;
+    sites += g.sites;
+  }
+  if (groups.size() > 1 && 8.5 * (double)groups.size() > 10.0 + (double)sites * 120.0 / 4.5e6)
+    return TH2D_ELAUNCH;  // (us per iteration) the step kernel is faster
   // control block: the workgroup table, per chain an iteration counter and a
   // set-up counter (a cache line each), the error flags, the XCD table, the
   // partials and the publication planes
@@ -1324,7 +1384,7 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   for (int c = 0; c < nch; c++)
     nparts += ((int)(((G[c].nt + RShape::TT - 1) / RShape::TT) *
                      ((G[c].nx + RShape::TX - 1) / RShape::TX)) + 7) & ~7;
-  const size_t ntab = ((size_t)grid + 31) & ~(size_t)31;
+  const size_t ntab = (wg_tile.size() + 31) & ~(size_t)31;
   const size_t nzero = 64 * (size_t)nch + 32;  // counters and error flags, zeroed
   const size_t nxcc = ((size_t)nreg + 31) & ~(size_t)31;
   const size_t pub_bytes = 6 * (size_t)nreg * RES_PUB * sizeof(c2<float>);  // 2 x (r, s, w)
@@ -1354,9 +1414,15 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   hipLaunchKernelGGL(k_zero_u32, dim3(std::min<int>(64, (int)(nzero + 255) / 256)), dim3(256), 0,
                      ctx->stream, ctl.bar, (int)nzero);
   TH2D_HIP(hipGetLastError());
-  void *args[] = {(void *)&a, (void *)&ctl};
-  TH2D_HIP(hipLaunchCooperativeKernel(k_cg_resident<float>, dim3(grid), dim3(RShape::NT), args, 0,
-                                      ctx->stream));
+  bool mu0 = true;
+  for (int c = 0; c < nch; c++) mu0 &= G[c].ep == 1.0 && G[c].em == 1.0;
+  const int *tab0 = ctl.wg_tile;
+  for (const ResGroup &g : groups) {  // one launch per group (each chain in one group)
+    ctl.wg_tile = tab0 + g.tab;
+    void *args[] = {(void *)&a, (void *)&ctl};
+    TH2D_HIP(hipLaunchCooperativeKernel(mu0 ? k_cg_resident<float, true> : k_cg_resident<float, false>,
+                                        dim3(g.grid), dim3(RShape::NT), args, 0, ctx->stream));
+  }
   // results and the two flags in one copy
   const size_t rb = sizeof(CgResult) * nch;
   TH2D_TRY(th2d_host_reserve(ctx->redh, rb + 64));
@@ -1373,7 +1439,7 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   ctx->cg_last_tile[0] = RShape::TT;
   ctx->cg_last_tile[1] = RShape::TX;
   ctx->cg_last_resident = 1;
-  return finish_solve<float>(ctx, nch, G, a, ch, h, 1, xout, res);
+  return finish_solve<float>(ctx, nch, G, a, ch, h, (long)groups.size(), xout, res);
 }
 
 template <typename T>
