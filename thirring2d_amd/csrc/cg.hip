@@ -469,7 +469,7 @@ struct ResCtl {
   int *err;          // set by a wait that timed out
   int *misplaced;    // set when a chain's regions do not share an XCD
   int *xcc;          // [ntiles] XCD of the workgroup of each region
-  double *part;      // [2][nparts][4] partials; a chain's start on a 256-B boundary
+  double *part;      // [2][nparts][waves][4] partials; a chain's start on a 2-KB boundary
   void *pub;         // [2][3][ntiles][RES_PUB] published boundary layers of r, s, w
                      // per parity (res_layout.h)
   int nparts;        // partial slots per parity
@@ -680,10 +680,11 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   };
   // partial slots of 4 doubles: (r.r, r.w) each iteration, and in the
   // set-up of a continued pass also (r.s, p.s) of the carried p, s
-  const unsigned o_part[2] = {boff(ctl.part) + (unsigned)(32 * ch.pad),
-                              boff(ctl.part) + (unsigned)(32 * (ctl.nparts + ch.pad))};
+  constexpr int NW = NT / 64;  // waves per workgroup: partial slots per region
+  const unsigned o_part[2] = {boff(ctl.part) + (unsigned)(32 * NW * ch.pad),
+                              boff(ctl.part) + (unsigned)(32 * NW * (ctl.nparts + ch.pad))};
   const unsigned o_bar = boff(ctl.bar + 32 * tl.chain);
-  const unsigned my_part = (unsigned)(32 * (tau - ch.tbeg));
+  const unsigned my_part = (unsigned)(32 * NW * (tau - ch.tbeg));
 
   // links and r_0 = b on the region and its halo
   for (int e = tid; e < E2; e += NT) {
@@ -855,7 +856,9 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
         sacc32 += p[q].x * s[q].x + p[q].y * s[q].y;
       }
     }
-    // 4. the region's partial (fixed order: lanes, then waves)
+    // 4. the partials of each wave (fixed lane order), stored by its lane 0:
+    // a chain's partials are summed by the readers, so no cross-wave
+    // reduction (LDS round trip and barrier) is needed here
     const int npart = setup && cont ? 4 : 2;
     {
       const double g = wave_sum((double)gacc32), d = wave_sum((double)dacc32);
@@ -865,30 +868,14 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
         ps = wave_sum((double)sacc32);
       }
       if ((tid & 63) == 0) {
-        red[4 * (tid >> 6)] = g;
-        red[4 * (tid >> 6) + 1] = d;
-        red[4 * (tid >> 6) + 2] = rs;
-        red[4 * (tid >> 6) + 3] = ps;
-      }
-    }
-    __syncthreads();
-    if (tid < 64) {  // the waves' sums in lanes 0..NT/64-1 of wave 0, fixed butterfly order
-      constexpr int NW = NT / 64;
-      static_assert((NW & (NW - 1)) == 0 && NW <= 64, "waves per workgroup");
-      double acc[4] = {0.0, 0.0, 0.0, 0.0};
-      if (tid < NW)
-        for (int u = 0; u < 4; u++) acc[u] = red[4 * tid + u];
-#pragma unroll
-      for (int off = 1; off < NW; off <<= 1) {
-        acc[0] += __shfl_xor(acc[0], off, 64);
-        acc[1] += __shfl_xor(acc[1], off, 64);
+        const unsigned mp = my_part + 32u * (unsigned)(tid >> 6);
+        M.std_(o_part[ns], mp, g);
+        M.std_(o_part[ns], mp + 8, d);
         if (npart == 4) {
-          acc[2] += __shfl_xor(acc[2], off, 64);
-          acc[3] += __shfl_xor(acc[3], off, 64);
+          M.std_(o_part[ns], mp + 16, rs);
+          M.std_(o_part[ns], mp + 24, ps);
         }
       }
-      if (tid == 0)
-        for (int u = 0; u < npart; u++) M.std_(o_part[ns], my_part + 8 * u, acc[u]);
     }
     slot = ns;
     setup = false;
@@ -902,7 +889,7 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
     const int jn = j + 1;
     const bool first_cont = jn == 0 && cont;
     double gam = 0.0, del = 0.0, rho = 0.0, sig = 0.0;
-    for (int k = tid & 63; k < ntc; k += 64) {
+    for (int k = tid & 63; k < ntc * NW; k += 64) {
       gam += M.ldd(o_part[slot], 32u * k);
       del += M.ldd(o_part[slot], 32u * k + 8);
       if (first_cont) {
@@ -1388,7 +1375,8 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   const size_t nzero = 64 * (size_t)nch + 32;  // counters and error flags, zeroed
   const size_t nxcc = ((size_t)nreg + 31) & ~(size_t)31;
   const size_t pub_bytes = 6 * (size_t)nreg * RES_PUB * sizeof(c2<float>);  // 2 x (r, s, w)
-  const size_t extra_bytes = 4 * (ntab + nzero + nxcc) + 64 * (size_t)nparts + pub_bytes;
+  constexpr size_t NW = RShape::NT / 64;  // partial slots of a region: one per wave
+  const size_t extra_bytes = 4 * (ntab + nzero + nxcc) + 64 * NW * (size_t)nparts + pub_bytes;
   CgBatch<float> a;
   std::vector<CgChain> ch;
   void *extra = nullptr;
@@ -1404,7 +1392,7 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   ctl.misplaced = ctl.err + 1;
   ctl.xcc = ctl.err + 32;
   ctl.part = (double *)(ctl.xcc + nxcc);  // 128-B aligned: every count above is a multiple of 32
-  ctl.pub = (char *)ctl.part + 64 * (size_t)nparts;  // 512-B aligned: nparts is a multiple of 8
+  ctl.pub = (char *)ctl.part + 64 * NW * (size_t)nparts;  // 512-B aligned: nparts is a multiple of 8
   ctl.nparts = nparts;
   const size_t blk = (size_t)((char *)extra + extra_bytes - (char *)a.x);
   if (blk >= (size_t)1 << 31) return TH2D_ELAUNCH;  // buffer offsets are 32-bit
