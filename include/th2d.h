@@ -165,13 +165,17 @@ int th2d_set_cg_max_blocks(th2d_ctx *ctx, int n);
 int th2d_set_cg_autotune(th2d_ctx *ctx, int on);
 /* Tile shape used by the last solve on this context. */
 int th2d_cg_last_tile(th2d_ctx *ctx, int *tt, int *tx);
-/* Resident FP32 solver (1 on, default; 0 off): an FP32 inner solve whose
- * lattices cut into no more 64x64 regions than the device holds resident
+/* Resident FP32 solver (1 on, default; 0 off): an FP32 inner solve runs as
+ * cooperative launches that keep every 64x64 region's vectors in registers,
+ * one launch per group of chains that fits the device's resident
  * workgroups (one per CU on MI355X: 256 regions, e.g. 16 lattices of
- * 256^2) runs as one cooperative launch that keeps every region's vectors
- * in registers; other solves use the step kernel.  A tile set with
- * th2d_set_cg_tile also selects the step kernel.  th2d_cg_last_resident
- * tells which kernel the last solve used (1 resident). */
+ * 256^2).  The step kernel solves the batch instead when a lattice is
+ * larger than one XCD holds (32 regions), when a lattice is cut into
+ * several regions with a clipped edge region (a side above 64 that is not
+ * a multiple of 64), or when a cost model prices it faster.  A tile set
+ * with th2d_set_cg_tile also selects the step kernel.
+ * th2d_cg_last_resident tells which kernel the last solve used (1
+ * resident). */
 int th2d_set_cg_resident(th2d_ctx *ctx, int on);
 /* Reliable updates of the mixed solver (0 off, default; 1 on).  Each FP64
  * pass replaces the FP32 inner solver's residual with the true residual.
