@@ -126,6 +126,28 @@ def test_lu_mfma_matches_valu(bag, native):
             assert np.max(np.abs(a - b)) / np.max(np.abs(b)) <= tol
 
 
+@pytest.mark.parametrize("reg", [1, 0])
+def test_lu_panels_match_numpy(bag, native, reg):
+    """Both LU panel factorisations (16 columns in registers with the row
+    interchanges applied afterwards; 32 columns with whole-row swaps as they
+    go) give numpy's log|det| and sign, on orders from 2 to 2304 in one
+    batch."""
+    rng = np.random.default_rng(21)
+    ns = [2, 16, 17, 48, 300, 1024, 1025, 2304]
+    mats = _mats(rng, ns)
+    native.lib().th2d_set_lu_panel(reg)
+    try:
+        db = bag.DenseBatch(mats, native.PREC_F64).lu()
+        la, sg = db.logdet()
+    finally:
+        native.lib().th2d_set_lu_panel(1)
+    assert np.all(db.infos() == 0)
+    for M, l, s in zip(mats, la, sg):
+        s_np, l_np = np.linalg.slogdet(M)
+        assert s == int(s_np)
+        assert abs(l - l_np) <= 1e-10 * max(1.0, abs(l_np))
+
+
 def test_lu_logdet_f32_bound(bag, native):
     """FP32-MFMA LU: log|det| within the stated bound of the FP64 value."""
     rng = np.random.default_rng(13)

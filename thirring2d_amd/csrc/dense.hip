@@ -193,50 +193,210 @@ __global__ void __launch_bounds__(256) k_lu_panel(T *base, BatchView bv, int *pi
   }
 }
 
-// U12 = L11^{-1} A12 on rows k..k+jb-1, columns c0.. (unit lower L11).
+// Panel of width PNB held in registers: one 1024-thread block per matrix,
+// row k + t + q*PT of the panel in thread t, slot q < PR (orders up to
+// PR*PT).  Per column: the pivot by a wave argmax and a pass over the 16
+// wave results (largest |a_ij|, ties to the smaller row, as k_lu_panel),
+// the exchange of the two rows' panel parts through LDS, the scaling of the
+// column and the rank-1 update of the panel's later columns, all on
+// registers.  Only the panel's columns are swapped here; k_lu_swap applies
+// the panel's interchanges to the other columns afterwards, in order, so
+// the factors equal those of whole-row swaps.
+constexpr int PNB = 16;   // register panel width
+constexpr int PT = 1024;  // threads of the register panel
+constexpr int PR = 3;     // panel rows per thread
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_lu_trsm(T *base, BatchView bv, int k)
+__global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int *pivb, int *info,
+                                                     int k)
 {
-  __shared__ T sL[NB * NB];
+  __shared__ double sval[PT / 64];
+  __shared__ int sidx[PT / 64];
+  __shared__ T prow[PNB], xrow[PNB];
+  const int b = blockIdx.x;
+  const int n = bv.n[b];
+  if (k >= n) return;
+  T *A = base + bv.off[b];
+  int *piv = pivb + bv.poff[b];
+  const int jb = min(PNB, n - k), tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  T a[PR][PNB];
+#pragma unroll
+  for (int q = 0; q < PR; q++) {
+    const int i = k + tid + q * PT;
+#pragma unroll
+    for (int c = 0; c < PNB; c++) a[q][c] = (i < n && c < jb) ? A[(long)i * n + k + c] : (T)0;
+  }
+#pragma clang loop unroll(full)
+  for (int j = 0; j < PNB; j++) {
+    if (j >= jb) continue;  // (a full unroll keeps a[][] in registers)
+    const int gj = k + j;
+    double best = -1.0;
+    int bi = n;
+#pragma unroll
+    for (int q = 0; q < PR; q++) {
+      const int i = k + tid + q * PT;
+      const double v = fabs((double)a[q][j]);
+      if (i >= gj && i < n && v > best) {  // rows of a thread ascend: ties keep the first
+        best = v;
+        bi = i;
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double v2 = __shfl_xor(best, off, 64);
+      const int i2 = __shfl_xor(bi, off, 64);
+      if (v2 > best || (v2 == best && i2 < bi)) {
+        best = v2;
+        bi = i2;
+      }
+    }
+    if (lane == 0) {
+      sval[wv] = best;
+      sidx[wv] = bi;
+    }
+    __syncthreads();
+    best = sval[0];
+    bi = sidx[0];
+    for (int w = 1; w < PT / 64; w++) {
+      const double v2 = sval[w];
+      const int i2 = sidx[w];
+      if (v2 > best || (v2 == best && i2 < bi)) {
+        best = v2;
+        bi = i2;
+      }
+    }
+    const int p = bi;  // >= gj: row gj itself is a candidate
+    if (tid == 0) {
+      piv[gj] = p;
+      if (best == 0.0 && info[b] == 0) info[b] = gj + 1;
+    }
+    // exchange rows gj and p of the panel: the pivot row to prow, the
+    // displaced row to xrow
+#pragma unroll
+    for (int q = 0; q < PR; q++) {
+      const int i = k + tid + q * PT;
+      if (i == p) {
+#pragma unroll
+        for (int c = 0; c < PNB; c++) prow[c] = a[q][c];
+      }
+      if (i == gj) {
+#pragma unroll
+        for (int c = 0; c < PNB; c++) xrow[c] = a[q][c];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PR; q++) {
+      const int i = k + tid + q * PT;
+      if (i == gj) {
+#pragma unroll
+        for (int c = 0; c < PNB; c++) a[q][c] = prow[c];
+      } else if (i == p) {
+#pragma unroll
+        for (int c = 0; c < PNB; c++) a[q][c] = xrow[c];
+      }
+    }
+    // L column and rank-1 update of the later panel columns
+    const T pv = prow[j];
+#pragma unroll
+    for (int q = 0; q < PR; q++) {
+      const int i = k + tid + q * PT;
+      if (i > gj && i < n) {
+        T l = a[q][j];
+        if (pv != (T)0) l /= pv;
+        a[q][j] = l;
+#pragma unroll
+        for (int c = 0; c < PNB; c++)  // constant trip count: indices fold after unrolling
+          if (c > j) a[q][c] -= l * prow[c];
+      }
+    }
+    __syncthreads();  // prow, xrow, sval are rewritten by the next column
+  }
+#pragma unroll
+  for (int q = 0; q < PR; q++) {
+    const int i = k + tid + q * PT;
+    if (i < n)
+#pragma unroll
+      for (int c = 0; c < PNB; c++)
+        if (c < jb) A[(long)i * n + k + c] = a[q][c];
+  }
+}
+
+// The row interchanges of the panel at k (pivots k..k+jb-1, in order) on
+// every column outside it: one thread per column.
+template <typename T>
+__global__ void __launch_bounds__(256) k_lu_swap(T *base, BatchView bv, const int *pivb, int k,
+                                                 int nb)
+{
   const int b = blockIdx.y;
   const int n = bv.n[b];
   if (k >= n) return;
-  const int jb = min(NB, n - k);
-  if (k + jb >= n) return;
+  const int jb = min(nb, n - k);
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= n || (c >= k && c < k + jb)) return;
+  T *A = base + bv.off[b];
+  const int *piv = pivb + bv.poff[b];
+  for (int j = k; j < k + jb; j++) {
+    const int p = piv[j];
+    if (p != j) {
+      const T t0 = A[(long)j * n + c];
+      A[(long)j * n + c] = A[(long)p * n + c];
+      A[(long)p * n + c] = t0;
+    }
+  }
+}
+
+// U12 = L11^{-1} A12 on rows k..k+jb-1 of the columns c0 <= c < c1 (all
+// from k + jb when c1 <= 0), unit lower L11 of order jb <= JB.  One thread
+// per column; its column of U12 lives in LDS (sx[r][thread]), so the
+// recurrence needs neither unrolling nor registers per row.
+template <typename T, int JB>
+__global__ void __launch_bounds__(256) k_lu_trsm(T *base, BatchView bv, int k, int c0, int c1)
+{
+  __shared__ T sL[JB * JB];
+  __shared__ T sx[JB * 256];
+  const int b = blockIdx.y;
+  const int n = bv.n[b];
+  if (k >= n) return;
+  const int jb = min(JB, n - k);
+  const int cb = c1 > 0 ? c0 : k + jb, ce = c1 > 0 ? min(c1, n) : n;
+  if (cb >= ce) return;
   T *A = base + bv.off[b];
   for (int e = threadIdx.x; e < jb * jb; e += 256) {
-    int r = e / jb, q = e - r * jb;
-    sL[r * NB + q] = A[(long)(k + r) * n + k + q];
+    const int r = e / jb, q = e - r * jb;
+    sL[r * JB + q] = A[(long)(k + r) * n + k + q];
   }
   __syncthreads();
-  const int c = k + jb + blockIdx.x * 256 + threadIdx.x;
-  if (c >= n) return;
-  T x[NB];
-  for (int r = 0; r < jb; r++) x[r] = A[(long)(k + r) * n + c];
+  const int t = threadIdx.x, c = cb + blockIdx.x * 256 + t;
+  if (c >= ce) return;
+  for (int r = 0; r < jb; r++) sx[r * 256 + t] = A[(long)(k + r) * n + c];
   for (int r = 1; r < jb; r++) {
-    T s = x[r];
-    for (int q = 0; q < r; q++) s -= sL[r * NB + q] * x[q];
-    x[r] = s;
+    T s = sx[r * 256 + t];
+    for (int q = 0; q < r; q++) s -= sL[r * JB + q] * sx[q * 256 + t];
+    sx[r * 256 + t] = s;
+    A[(long)(k + r) * n + c] = s;
   }
-  for (int r = 1; r < jb; r++) A[(long)(k + r) * n + c] = x[r];
 }
 
 // A22 -= L21 U12 for the trailing block after panel k.
+// (columns k + jb .. c1 - 1 only when c1 > 0)
 template <typename T, bool MF>
-__global__ void __launch_bounds__(256) k_lu_update(T *base, BatchView bv, int k)
+__global__ void __launch_bounds__(256) k_lu_update(T *base, BatchView bv, int k, int nb, int c1)
 {
   __shared__ T sA[BM * (BK + 1)];
   __shared__ T sB[BK * (BN + 1)];
   const int b = blockIdx.z;
   const int n = bv.n[b];
-  const int jb = min(NB, n - k);
-  const int m = n - k - jb;
-  if (k >= n || m <= 0) return;
+  if (k >= n) return;
+  const int jb = min(nb, n - k);
+  const int m = n - k - jb;                                 // rows
+  const int mc = (c1 > 0 ? min(c1, n) : n) - (k + jb);      // columns
+  if (m <= 0 || mc <= 0) return;
   const int ti = blockIdx.y * BM, tj = blockIdx.x * BN;
-  if (ti >= m || tj >= m) return;
+  if (ti >= m || tj >= mc) return;
   T *A = base + bv.off[b];
   const long o = (long)(k + jb) * n + (k + jb);
-  gemm_tile<T, MF>(A + (long)(k + jb) * n + k, n, A + (long)k * n + (k + jb), n, A + o, n, m, m,
+  gemm_tile<T, MF>(A + (long)(k + jb) * n + k, n, A + (long)k * n + (k + jb), n, A + o, n, m, mc,
                    jb, (T)-1, ti, tj, sA, sB);
 }
 
@@ -388,6 +548,7 @@ __global__ void k_f64_to_f32_n(float *d, const double *s, long n)
 }
 
 static int g_use_mfma = -1;
+static int g_lu_reg_panel = 1;  // register panel when it fits (th2d_set_lu_panel)
 bool use_mfma()
 {
   if (g_use_mfma < 0) {
@@ -405,16 +566,44 @@ int lu_run(th2d_ctx *c, const th2d_batch *bt, T *A, int *piv, int *info)
   BatchView bv = view(bt);
   TH2D_TRY(th2d_memset(c, info, 0, sizeof(int) * bt->count));
   const bool mf = use_mfma();
-  for (int k = 0; k < bt->max_n; k += NB) {
-    hipLaunchKernelGGL(k_lu_panel<T>, dim3(bt->count), dim3(256), 0, c->stream, A, bv, piv, info,
-                       k);
-    int rest = bt->max_n - k - NB;
+  // panels in registers when every matrix's rows fit the block (PR rows
+  // per thread): blocks of OB = 2 PNB columns, each factored as two
+  // register panels (the second after the first's update of its columns),
+  // then one trailing update of width OB; else the LDS-reduction panel of
+  // width NB
+  const bool reg = bt->max_n <= PR * PT && g_lu_reg_panel;
+  constexpr int OB = 2 * PNB;
+  static_assert(OB == NB, "the outer block is the LDS panel's width");
+  const int N = bt->max_n;
+  for (int k = 0; k < N; k += NB) {
+    if (reg) {
+      for (int kk = k; kk < std::min(k + OB, N); kk += PNB) {
+        hipLaunchKernelGGL(k_lu_panel_reg<T>, dim3(bt->count), dim3(PT), 0, c->stream, A, bv, piv,
+                           info, kk);
+        hipLaunchKernelGGL(k_lu_swap<T>, dim3((N + 255) / 256, bt->count), dim3(256), 0,
+                           c->stream, A, bv, piv, kk, PNB);
+        const int cb = kk + PNB, ce = std::min(k + OB, N);  // the block's later columns
+        if (cb < ce) {
+          hipLaunchKernelGGL((k_lu_trsm<T, PNB>), dim3(1, bt->count), dim3(256), 0, c->stream, A,
+                             bv, kk, cb, ce);
+          dim3 g((ce - cb + BN - 1) / BN, (N - cb + BM - 1) / BM, bt->count);
+          if (mf)
+            hipLaunchKernelGGL((k_lu_update<T, true>), g, dim3(256), 0, c->stream, A, bv, kk, PNB, ce);
+          else
+            hipLaunchKernelGGL((k_lu_update<T, false>), g, dim3(256), 0, c->stream, A, bv, kk, PNB, ce);
+        }
+      }
+    } else {
+      hipLaunchKernelGGL(k_lu_panel<T>, dim3(bt->count), dim3(256), 0, c->stream, A, bv, piv,
+                         info, k);
+    }
+    const int rest = N - k - NB;
     if (rest > 0) {
-      hipLaunchKernelGGL(k_lu_trsm<T>, dim3((rest + 255) / 256, bt->count), dim3(256), 0,
-                         c->stream, A, bv, k);
+      hipLaunchKernelGGL((k_lu_trsm<T, NB>), dim3((rest + 255) / 256, bt->count), dim3(256), 0,
+                         c->stream, A, bv, k, 0, 0);
       dim3 g((rest + BN - 1) / BN, (rest + BM - 1) / BM, bt->count);
-      if (mf) hipLaunchKernelGGL((k_lu_update<T, true>), g, dim3(256), 0, c->stream, A, bv, k);
-      else hipLaunchKernelGGL((k_lu_update<T, false>), g, dim3(256), 0, c->stream, A, bv, k);
+      if (mf) hipLaunchKernelGGL((k_lu_update<T, true>), g, dim3(256), 0, c->stream, A, bv, k, NB, 0);
+      else hipLaunchKernelGGL((k_lu_update<T, false>), g, dim3(256), 0, c->stream, A, bv, k, NB, 0);
     }
   }
   TH2D_HIP(hipGetLastError());
@@ -533,6 +722,12 @@ extern "C" int th2d_gemm_batched(th2d_ctx *c, void *base, int precision,
 extern "C" int th2d_set_dense_mfma(int on)
 {
   g_use_mfma = on ? 1 : 0;
+  return TH2D_OK;
+}
+
+extern "C" int th2d_set_lu_panel(int reg)
+{
+  g_lu_reg_panel = reg ? 1 : 0;
   return TH2D_OK;
 }
 
