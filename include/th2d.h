@@ -169,13 +169,15 @@ int th2d_cg_last_tile(th2d_ctx *ctx, int *tt, int *tx);
  * cooperative launches that keep every 64x64 region's vectors in registers,
  * one launch per group of chains that fits the device's resident
  * workgroups (one per CU on MI355X: 256 regions, e.g. 16 lattices of
- * 256^2).  The step kernel solves the batch instead when a lattice is
- * larger than one XCD holds (32 regions), when a lattice is cut into
- * several regions with a clipped edge region (a side above 64 that is not
- * a multiple of 64), or when a cost model prices it faster.  A tile set
- * with th2d_set_cg_tile also selects the step kernel.
- * th2d_cg_last_resident tells which kernel the last solve used (1
- * resident). */
+ * 256^2).  A lattice the resident kernel does not take -- larger than one
+ * XCD holds (32 regions), or cut into several regions with a clipped edge
+ * region (a side above 64 that is not a multiple of 64) -- is solved by the
+ * step kernel; a batch that mixes such lattices with others is split
+ * between the two kernels.  The whole batch goes to the step kernel when a
+ * cost model prices it faster, or when a tile is set with
+ * th2d_set_cg_tile.  th2d_cg_last_resident tells which kernel the last
+ * inner solve used: 0 the step kernel, 1 the resident kernel, 2 both (a
+ * split batch). */
 int th2d_set_cg_resident(th2d_ctx *ctx, int on);
 /* Reliable updates of the mixed solver (0 off, default; 1 on).  Each FP64
  * pass replaces the FP32 inner solver's residual with the true residual.

@@ -417,13 +417,42 @@ def test_cg_resident_groups(hmc, oracle):
         assert abs(it1 - it0) <= max(3, it0 // 10)
 
 
-def test_cg_resident_declines_clipped_regions(hmc, oracle):
-    """A batch with a lattice cut into several regions with a clipped edge
-    region (96^2: regions of 64 and 32 sites a side) is solved by the step
-    kernel; the solution reaches the true residual."""
+def test_cg_resident_splits_clipped_regions(hmc, oracle):
+    """A batch with lattices cut into several regions with a clipped edge
+    region (96^2: regions of 64 and 32 sites a side) is split: those chains
+    in the step kernel, the others (128^2, 32^2, 64^2) in the resident
+    kernel.  Every chain reaches the true residual, with the iteration
+    counts of the step kernel alone within 10%."""
     from thirring2d_amd import _native as N
-    setups = [_setup(hmc, oracle, nt, nx, 600 + k, m=0.1)
-              for k, (nt, nx) in enumerate([(96, 96), (64, 64)])]
+    dims = [(96, 96), (128, 128), (32, 32), (96, 96), (64, 64)]
+    setups = [_setup(hmc, oracle, nt, nx, 600 + k, m=0.1) for k, (nt, nx) in enumerate(dims)]
+    batch = hmc.ChainBatch([s[0] for s in setups])
+    ctx = setups[0][0].ctx
+    runs = {}
+    try:
+        for on in (1, 0):
+            assert N.lib().th2d_set_cg_resident(ctx.ptr, on) == 0
+            xs, infos = batch.fmdm_invert_cg([s[3] for s in setups],
+                                             hmc.cg_params(tol_rel=1e-10, precision=N.PREC_MIXED))
+            assert _last_resident(ctx) == (2 if on else 0)
+            runs[on] = (xs, infos)
+    finally:
+        N.lib().th2d_set_cg_resident(ctx.ptr, 1)
+    for k, (lat, L, A, b, _) in enumerate(setups):
+        x, inf = runs[1][0][k], runs[1][1][k]
+        assert inf.status == 0
+        r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
+        assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
+        it1, it0 = inf.iterations, runs[0][1][k].iterations
+        assert abs(it1 - it0) <= max(3, it0 // 10)
+
+
+def test_cg_resident_declines_clipped_regions(hmc, oracle):
+    """A batch of only lattices cut into several regions with a clipped
+    edge region (96^2) is solved by the step kernel."""
+    from thirring2d_amd import _native as N
+    setups = [_setup(hmc, oracle, nt, nx, 650 + k, m=0.1)
+              for k, (nt, nx) in enumerate([(96, 96), (96, 160)])]
     batch = hmc.ChainBatch([s[0] for s in setups])
     xs, infos = batch.fmdm_invert_cg([s[3] for s in setups],
                                      hmc.cg_params(tol_rel=1e-10, precision=N.PREC_MIXED))

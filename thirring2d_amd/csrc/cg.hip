@@ -1029,6 +1029,7 @@ struct InnerResult {
   double rr;
   int status;
   double gam, alpha;  // resident solver: state for a reliable update
+  int resident;       // solved by the resident kernel
 };
 
 // Carried Krylov state of a chain between passes of the mixed solver (a
@@ -1040,6 +1041,8 @@ struct Carry {
 
 // Batched inner solve: A_c y_c = rhs_c from y_c = 0, stop r.r < stop_c.
 // The solutions are the internal x vectors, returned in xout[c].
+// force_shape: -1 automatic (resident kernel where it applies), -2 step
+// kernel with an automatic tile shape, >= 0 that step-kernel shape.
 template <typename T>
 int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
                 const c2<T> *const *rhs,
@@ -1230,6 +1233,7 @@ int finish_solve(th2d_ctx *ctx, int nch, const Geo *G, const CgBatch<T> &a,
     res[c].status = h[c].status;
     res[c].gam = h[c].gam_old;
     res[c].alpha = h[c].alpha_old;
+    res[c].resident = 0;
     xout[c] = a.x + ch[c].voff;
   }
   return TH2D_OK;
@@ -1427,7 +1431,68 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   ctx->cg_last_tile[0] = RShape::TT;
   ctx->cg_last_tile[1] = RShape::TX;
   ctx->cg_last_resident = 1;
-  return finish_solve<float>(ctx, nch, G, a, ch, h, (long)groups.size(), xout, res);
+  TH2D_TRY(finish_solve<float>(ctx, nch, G, a, ch, h, (long)groups.size(), xout, res));
+  for (int c = 0; c < nch; c++) res[c].resident = 1;
+  return TH2D_OK;
+}
+
+// Whether the resident solver takes a lattice: eligible, and no more
+// regions than one XCD holds.
+bool res_takes(const Geo &g, int cap)
+{
+  const long r = (long)((g.nt + RShape::TT - 1) / RShape::TT) * ((g.nx + RShape::TX - 1) / RShape::TX);
+  return res_eligible(g) && r <= cap / kNumXcd;
+}
+
+template <typename T>
+int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
+                const c2<T> *const *rhs, const c2<T> *const *K, const double *stop_rr,
+                const int *max_iter, c2<T> **xout, InnerResult *res, int force_shape,
+                const Carry *carry);
+
+// A batch split between the kernels: the chains `ra` in the resident
+// kernel (block in `buf`), the chains `rb` in the step kernel (block in the
+// context's second FP32 buffer), one after the other on the stream.
+// TH2D_ELAUNCH when the resident kernel declines its part.
+int split_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, const Geo *G,
+                const c2<float> *const *rhs, const c2<float> *const *K, const double *stop_rr,
+                const int *max_iter, c2<float> **xout, InnerResult *res, const Carry *carry,
+                const std::vector<int> &ra, const std::vector<int> &rb)
+{
+  for (int part = 0; part < 2; part++) {
+    const std::vector<int> &ix = part == 0 ? ra : rb;
+    const int n = (int)ix.size();
+    std::vector<Geo> g(n);
+    std::vector<const c2<float> *> r(n), k(n);
+    std::vector<double> st(n);
+    std::vector<int> mi(n);
+    std::vector<c2<float> *> xo(n);
+    std::vector<InnerResult> ir(n);
+    std::vector<Carry> cy(n);
+    for (int q = 0; q < n; q++) {
+      const int c = ix[q];
+      g[q] = G[c];
+      r[q] = rhs[c];
+      k[q] = K[c];
+      st[q] = stop_rr[c];
+      mi[q] = max_iter[c];
+      if (carry) cy[q] = carry[c];
+    }
+    if (part == 0) {
+      const int rc = resident_solve(ctx, buf, n, g.data(), r.data(), k.data(), st.data(), mi.data(),
+                                    xo.data(), ir.data(), carry ? cy.data() : nullptr);
+      if (rc != TH2D_OK) return rc;
+    } else {
+      TH2D_TRY(inner_solve<float>(ctx, ctx->cg32b, pass, n, g.data(), r.data(), k.data(), st.data(),
+                                  mi.data(), xo.data(), ir.data(), -2, nullptr));
+    }
+    for (int q = 0; q < n; q++) {
+      xout[ix[q]] = xo[q];
+      res[ix[q]] = ir[q];
+    }
+  }
+  ctx->cg_last_resident = 2;
+  return TH2D_OK;
 }
 
 template <typename T>
@@ -1437,8 +1502,16 @@ int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
                 InnerResult *res, int force_shape, const Carry *carry)
 {
   if constexpr (sizeof(T) == 4) {
-    if (force_shape < 0 && ctx->cg_resident && !(ctx->tile_t > 0 && ctx->tile_x > 0)) {
-      const int rc = resident_solve(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, xout, res, carry);
+    if (force_shape == -1 && ctx->cg_resident && !(ctx->tile_t > 0 && ctx->tile_x > 0)) {
+      // the chains the resident kernel takes; a batch with others is split
+      const int cap = res_capacity(ctx);
+      std::vector<int> ra, rb;
+      for (int c = 0; c < nch; c++) (res_takes(G[c], cap) ? ra : rb).push_back(c);
+      int rc = TH2D_ELAUNCH;
+      if (rb.empty())
+        rc = resident_solve(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, xout, res, carry);
+      else if (!ra.empty())
+        rc = split_solve(ctx, buf, pass, G, rhs, K, stop_rr, max_iter, xout, res, carry, ra, rb);
       if (rc != TH2D_ELAUNCH) return rc;
     }
   }
@@ -1722,7 +1795,7 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
                                   stop.data(), mi.data(), xo.data(), ir.data(), -1, cy.data()));
       for (int k = 0; k < na; k++) {  // (kept only if the pass stays consistent, below)
         const int c = idx[k];
-        carry_ok[c] = ctx->cg_last_resident && ir[k].status != TH2D_EBREAKDOWN;
+        carry_ok[c] = ir[k].resident && ir[k].status != TH2D_EBREAKDOWN;
         cgam[c] = ir[k].gam;
         calpha[c] = ir[k].alpha;
       }

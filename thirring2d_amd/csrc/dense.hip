@@ -210,9 +210,11 @@ template <typename T>
 __global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int *pivb, int *info,
                                                      int k)
 {
-  __shared__ double sval[PT / 64];
-  __shared__ int sidx[PT / 64];
-  __shared__ T prow[PNB], xrow[PNB];
+  // double-buffered by column parity: a buffer is rewritten two columns
+  // later, after every thread has passed the barriers between
+  __shared__ double svalb[2][PT / 64];
+  __shared__ int sidxb[2][PT / 64];
+  __shared__ T prowb[2][PNB], xrowb[2][PNB];
   const int b = blockIdx.x;
   const int n = bv.n[b];
   if (k >= n) return;
@@ -230,6 +232,9 @@ __global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int 
   for (int j = 0; j < PNB; j++) {
     if (j >= jb) continue;  // (a full unroll keeps a[][] in registers)
     const int gj = k + j;
+    double *sval = svalb[j & 1];
+    int *sidx = sidxb[j & 1];
+    T *prow = prowb[j & 1], *xrow = xrowb[j & 1];
     double best = -1.0;
     int bi = n;
 #pragma unroll
@@ -310,7 +315,6 @@ __global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int 
           if (c > j) a[q][c] -= l * prow[c];
       }
     }
-    __syncthreads();  // prow, xrow, sval are rewritten by the next column
   }
 #pragma unroll
   for (int q = 0; q < PR; q++) {

@@ -67,6 +67,7 @@ extern "C" int th2d_ctx_destroy(th2d_ctx *c)
   (void)hipStreamSynchronize(c->stream);
   buf_release(c->cg64);
   buf_release(c->cg32);
+  buf_release(c->cg32b);
   buf_release(c->cgmix);
   buf_release(c->red);
   buf_release(c->redh, true);

@@ -50,7 +50,8 @@ struct th2d_ctx {
   long klaunches = 0;
   double kbytes = 0.0;
   // scratch
-  th2d_buf cg64, cg32, cgmix, red, redh, hmc, force, bag, misc;
+  th2d_buf cg64, cg32, cg32b, cgmix, red, redh, hmc, force, bag, misc;  // cg32b: step-kernel
+                                                                       // part of a split batch
   th2d_buf bpart, btab[4];  // batched site operations: partials, descriptor tables
   // tile configuration of the persistent solver (0 = autotuned default)
   int tile_t = 0, tile_x = 0;
@@ -63,7 +64,7 @@ struct th2d_ctx {
   int cg_resident = 1;            // FP32 inner solves in the resident kernel when they fit
   int cg_reliable = 0;            // mixed solves continue the Krylov sequence across passes
   int res_capacity = -1;          // resident workgroups the device holds (-1: not queried)
-  int cg_last_resident = 0;       // the last inner solve ran resident
+  int cg_last_resident = 0;       // the last inner solve: 0 step kernel, 1 resident, 2 both
   std::map<unsigned long long, int> cg_tuned[2];  // signature -> shape (FP32, FP64)
 };
 
