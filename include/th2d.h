@@ -179,6 +179,15 @@ int th2d_cg_last_tile(th2d_ctx *ctx, int *tt, int *tx);
  * inner solve used: 0 the step kernel, 1 the resident kernel, 2 both (a
  * split batch). */
 int th2d_set_cg_resident(th2d_ctx *ctx, int on);
+/* Step-kernel launches as captured graphs (0 off, default; 1 on): the
+ * step kernel's iterations are issued as replays of a hipGraph holding 8
+ * launches, captured once per batch layout on a private stream and cached
+ * on the context (at most 16).  A solve then runs a whole number of
+ * replays; the launches after every chain has stopped find nothing to do.
+ * Off by default: on MI355X the replays measured 1-6% slower than eager
+ * launches (C4, batched and single-chain FP64 HMC), whose gaps the
+ * kernels' own duration already hides.  Replaces no reference function. */
+int th2d_set_cg_graph(th2d_ctx *ctx, int on);
 /* Reliable updates of the mixed solver (0 off, default; 1 on).  Each FP64
  * pass replaces the FP32 inner solver's residual with the true residual.
  * Off, every pass restarts CG from p = 0 (defect correction).  On, a pass

@@ -463,6 +463,40 @@ def test_cg_resident_declines_clipped_regions(hmc, oracle):
         assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
 
 
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+def test_cg_step_graph_matches_eager(hmc, oracle, prec):
+    """The step kernel's launches replayed from a captured graph and issued
+    one by one give bitwise the same solutions and iteration counts, over
+    three solves in a row with other work (the FP64 true-residual passes)
+    between the replays; the graph is captured once and reused."""
+    from thirring2d_amd import _native as N
+    dims = [(96, 96), (40, 56), (16, 16), (96, 160)]
+    setups = [_setup(hmc, oracle, nt, nx, 900 + k, m=0.1) for k, (nt, nx) in enumerate(dims)]
+    batch = hmc.ChainBatch([s[0] for s in setups])
+    ctx = setups[0][0].ctx
+    precision = N.PREC_F64 if prec == "f64" else N.PREC_MIXED
+    rng = np.random.default_rng(901)
+    rhs = [[rng.normal(size=nt * nx) + 1j * rng.normal(size=nt * nx) for nt, nx in dims]
+           for _ in range(3)]
+    out = {}
+    try:
+        for g in (1, 0):
+            assert N.lib().th2d_set_cg_graph(ctx.ptr, g) == 0
+            out[g] = [batch.fmdm_invert_cg(b, hmc.cg_params(tol_rel=1e-10, precision=precision))
+                      for b in rhs]
+    finally:
+        N.lib().th2d_set_cg_graph(ctx.ptr, 1)
+    for (xs1, inf1), (xs0, inf0) in zip(out[1], out[0]):
+        for x1, x0, i1, i0 in zip(xs1, xs0, inf1, inf0):
+            assert i1.status == 0 and i0.status == 0
+            assert i1.iterations == i0.iterations
+            assert np.array_equal(x1, x0)
+    # and the solutions are solutions
+    for (lat, L, A, _, _), x, b in zip(setups, out[1][2][0], rhs[2]):
+        r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
+        assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
+
+
 def test_cg_resident_falls_back_when_too_large(hmc, oracle):
     """A batch with a lattice larger than an XCD's resident workgroups
     (512^2 = 64 regions) runs in the step kernel."""

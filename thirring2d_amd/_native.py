@@ -115,6 +115,7 @@ SIGNATURES = {
     "th2d_cg_last_tile": (I, [P, C.POINTER(I), C.POINTER(I)]),
     "th2d_set_cg_autotune": (I, [P, I]),
     "th2d_set_cg_resident": (I, [P, I]),
+    "th2d_set_cg_graph": (I, [P, I]),
     "th2d_set_cg_reliable": (I, [P, I]),
     "th2d_cg_last_resident": (I, [P, C.POINTER(I)]),
     "th2d_set_dslash_variant": (I, [I]),

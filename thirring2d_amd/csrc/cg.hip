@@ -622,6 +622,29 @@ __device__ __forceinline__ void res_ring_pos(int q, int &et, int &ex)
   }
 }
 
+// y += s a b and y += s conj(a) b for FP32 complex values, as chains of
+// fused multiply-adds into y: written as y + (a b - c d) the compiler may
+// not reassociate, and spends a separate add per term.  s = +-1 folds into
+// the operand's sign.
+template <typename V>
+__device__ __forceinline__ void cfma(V &y, V a, V b, float s)
+{
+  const float ax = s * a.x, ay = s * a.y;
+  y.x = __builtin_fmaf(ax, b.x, y.x);
+  y.x = __builtin_fmaf(-ay, b.y, y.x);
+  y.y = __builtin_fmaf(ax, b.y, y.y);
+  y.y = __builtin_fmaf(ay, b.x, y.y);
+}
+template <typename V>
+__device__ __forceinline__ void cjfma(V &y, V a, V b, float s)
+{
+  const float ax = s * a.x, ay = s * a.y;
+  y.x = __builtin_fmaf(ax, b.x, y.x);
+  y.x = __builtin_fmaf(ay, b.y, y.x);
+  y.y = __builtin_fmaf(ax, b.y, y.y);
+  y.y = __builtin_fmaf(-ay, b.x, y.y);
+}
+
 // XCD of the calling workgroup (hardware register XCC_ID, bits 3:0).
 __device__ __forceinline__ int xcc_id()
 {
@@ -822,11 +845,12 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       // all nine loads in flight before the arithmetic: one LDS latency per
       // element (two waves per SIMD hide little)
       __builtin_amdgcn_sched_barrier(0);
+      // y = m r + e^mu K_t r(t+1) - e^-mu K_t(t-1)* r(t-1) + K_x r(x+1) - K_x(x-1)* r(x-1)
       V2 y = cscale(r0, m);
-      y = cadd(y, cscale(cmul(kt, rtp), epm));
-      y = csub(y, cscale(cjmul(ktd, rtm), emm));
-      y = cadd(y, cmul(kx, rxp));
-      y = csub(y, cjmul(kxd, rxm));
+      cfma(y, kt, rtp, epm);
+      cjfma(y, ktd, rtm, -emm);
+      cfma(y, kx, rxp, 1.0f);
+      cjfma(y, kxd, rxm, -1.0f);
       Y1[e] = y;
     }
     __syncthreads();
@@ -842,11 +866,12 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       const V2 kt = LKt[c2i], kx = LKx[c2i], ktd = LKt[c2i - W2], kxd = LKx[c2i - 1];
       const V2 y0 = Y1[c1], ytp = Y1[c1 + W1], ytm = Y1[c1 - W1], yxp = Y1[c1 + 1], yxm = Y1[c1 - 1];
       __builtin_amdgcn_sched_barrier(0);  // one LDS latency per element (as in 3b)
+      // w = m y - e^-mu K_t y(t+1) + e^mu K_t(t-1)* y(t-1) - K_x y(x+1) + K_x(x-1)* y(x-1)
       V2 v = cscale(y0, m);
-      v = csub(v, cscale(cmul(kt, ytp), emm));
-      v = cadd(v, cscale(cjmul(ktd, ytm), epm));
-      v = csub(v, cmul(kx, yxp));
-      v = cadd(v, cjmul(kxd, yxm));
+      cfma(v, kt, ytp, -emm);
+      cjfma(v, ktd, ytm, epm);
+      cfma(v, kx, yxp, -1.0f);
+      cjfma(v, kxd, yxm, 1.0f);
       w[q] = v;
       if ((pub >> q) & 1u) M.st2(o_w[ns], pub_at(lt, lx), v);
       gacc32 += r[q].x * r[q].x + r[q].y * r[q].y;
@@ -1024,6 +1049,50 @@ void launch_step(int shape, int nblocks, hipStream_t st, const CgBatch<T> &a, in
 }
 static_assert(kNumShapes == 11, "launch_step lists every shape");
 
+// A run of kStepGraph step launches (parities 0, 1, 0, ...) captured as a
+// graph, cached per context by the launch arguments (element size, tile
+// shape, grid and the batch descriptor's bytes); at most 16 are kept.  The
+// capture runs on a private stream of the context (the context's own
+// stream may be the null stream, which cannot be captured); the graph is
+// then launched into the context's stream.  *out stays null when the
+// capture is refused: the caller then launches eagerly.
+constexpr int kStepGraph = 8;
+template <typename T>
+int step_graph(th2d_ctx *ctx, int shape, int nblocks, const CgBatch<T> &a, hipGraphExec_t *out)
+{
+  *out = nullptr;
+  const int hdr[3] = {(int)sizeof(T), shape, nblocks};
+  std::vector<unsigned char> key(sizeof(hdr) + sizeof(a));
+  memcpy(key.data(), hdr, sizeof(hdr));
+  memcpy(key.data() + sizeof(hdr), &a, sizeof(a));
+  for (const auto &g : ctx->cg_graphs)
+    if (g.key == key) {
+      *out = g.exec;
+      return TH2D_OK;
+    }
+  if (!ctx->cap_stream)
+    TH2D_HIP(hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking));
+  if (hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return TH2D_OK;
+  }
+  for (int k = 0; k < kStepGraph; k++) launch_step(shape, nblocks, ctx->cap_stream, a, k & 1);
+  hipGraph_t graph = nullptr;
+  TH2D_HIP(hipStreamEndCapture(ctx->cap_stream, &graph));
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return TH2D_EHIP;
+  if (ctx->cg_graphs.size() >= 16) {
+    TH2D_HIP(hipStreamSynchronize(ctx->stream));
+    (void)hipGraphExecDestroy(ctx->cg_graphs.front().exec);
+    ctx->cg_graphs.erase(ctx->cg_graphs.begin());
+  }
+  ctx->cg_graphs.push_back(th2d_ctx::CgGraph{std::move(key), ex});
+  *out = ex;
+  return TH2D_OK;
+}
+
 struct InnerResult {
   int iters;
   double rr;
@@ -1130,6 +1199,7 @@ int build_batch(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T>
                 const Carry *carry = nullptr, const void *extra_init = nullptr,
                 size_t extra_init_bytes = 0)
 {
+  memset(&a, 0, sizeof(a));  // padding included: the bytes key captured graphs
   ch.assign(nch, CgChain{});
   std::vector<CgTile> tiles;
   long voff = 0;
@@ -1561,9 +1631,17 @@ int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
   int chunk = ctx->cg_chunk > 0 ? ctx->cg_chunk : (last > 0 ? last + 1 : 32);
   int most = 0;
   for (int c = 0; c < nch; c++) most = std::max(most, max_iter[c]);
+  hipGraphExec_t gx = nullptr;  // the captured run of step launches
+  if (ctx->cg_graph) TH2D_TRY(step_graph<T>(ctx, shape, nblocks, a, &gx));
   for (;;) {
-    for (int k = 0; k < chunk; k++)
-      launch_step(shape, nblocks, ctx->stream, a, (int)((launched + k) & 1));
+    if (gx) {  // whole runs (launched stays even; surplus launches find every chain done)
+      const int reps = (chunk + kStepGraph - 1) / kStepGraph;
+      for (int r = 0; r < reps; r++) TH2D_HIP(hipGraphLaunch(gx, ctx->stream));
+      chunk = reps * kStepGraph;
+    } else {
+      for (int k = 0; k < chunk; k++)
+        launch_step(shape, nblocks, ctx->stream, a, (int)((launched + k) & 1));
+    }
     TH2D_HIP(hipGetLastError());
     launched += chunk;
     TH2D_HIP(hipMemcpyAsync(ctx->redh.ptr, a.res, sizeof(CgResult) * nch, hipMemcpyDeviceToHost,
@@ -1590,6 +1668,13 @@ int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
 
 
 }  // namespace
+
+extern "C" int th2d_set_cg_graph(th2d_ctx *ctx, int on)
+{
+  if (!ctx) return TH2D_EARG;
+  ctx->cg_graph = on ? 1 : 0;
+  return TH2D_OK;
+}
 
 extern "C" int th2d_set_cg_tile(th2d_ctx *ctx, int tt, int tx)
 {

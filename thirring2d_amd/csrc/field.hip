@@ -65,6 +65,9 @@ extern "C" int th2d_ctx_destroy(th2d_ctx *c)
 {
   if (!c) return TH2D_OK;
   (void)hipStreamSynchronize(c->stream);
+  for (auto &g : c->cg_graphs) (void)hipGraphExecDestroy(g.exec);
+  c->cg_graphs.clear();
+  if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
   buf_release(c->cg64);
   buf_release(c->cg32);
   buf_release(c->cg32b);

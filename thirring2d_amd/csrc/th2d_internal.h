@@ -66,6 +66,15 @@ struct th2d_ctx {
   int res_capacity = -1;          // resident workgroups the device holds (-1: not queried)
   int cg_last_resident = 0;       // the last inner solve: 0 step kernel, 1 resident, 2 both
   std::map<unsigned long long, int> cg_tuned[2];  // signature -> shape (FP32, FP64)
+  // step-kernel launches captured as graphs (cg.hip), keyed by the launch
+  // arguments; cg_graph 0 (default) launches eagerly
+  int cg_graph = 0;
+  hipStream_t cap_stream = nullptr;  // private stream the graphs are captured on
+  struct CgGraph {
+    std::vector<unsigned char> key;
+    hipGraphExec_t exec;
+  };
+  std::vector<CgGraph> cg_graphs;
 };
 
 int th2d_buf_reserve(th2d_buf &b, size_t bytes);
