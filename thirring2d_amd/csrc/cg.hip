@@ -645,6 +645,34 @@ __device__ __forceinline__ void cjfma(V &y, V a, V b, float s)
   y.y = __builtin_fmaf(-ay, b.x, y.y);
 }
 
+// Wave64 total of a double by DPP moves within the wave (no LDS round
+// trips): pairs, quads (quad permutations), half-rows and rows (mirrors),
+// then rows 0+1 into row 1 and 2+3 into row 3 (broadcast of lane 15), and
+// rows 0..3 into row 3 (broadcast of lane 31); lane 63 then holds the total
+// in a fixed order, and readlane gives it to every lane.  Rows a
+// broadcast does not write add 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_d(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double wave_total(double v)
+{
+  v += dpp_d<0xB1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += dpp_d<0x4E, 0xf>(v);   // quad_perm [2,3,0,1]
+  v += dpp_d<0x141, 0xf>(v);  // row_half_mirror
+  v += dpp_d<0x140, 0xf>(v);  // row_mirror
+  v += dpp_d<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
+  v += dpp_d<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | lo);
+}
+
 // XCD of the calling workgroup (hardware register XCC_ID, bits 3:0).
 __device__ __forceinline__ int xcc_id()
 {
@@ -886,11 +914,11 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
     // reduction (LDS round trip and barrier) is needed here
     const int npart = setup && cont ? 4 : 2;
     {
-      const double g = wave_sum((double)gacc32), d = wave_sum((double)dacc32);
+      const double g = wave_total((double)gacc32), d = wave_total((double)dacc32);
       double rs = 0.0, ps = 0.0;
       if (npart == 4) {
-        rs = wave_sum((double)racc32);
-        ps = wave_sum((double)sacc32);
+        rs = wave_total((double)racc32);
+        ps = wave_total((double)sacc32);
       }
       if ((tid & 63) == 0) {
         const unsigned mp = my_part + 32u * (unsigned)(tid >> 6);
@@ -932,11 +960,11 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
         hs[h] = M.ld2(o_s[slot], ri[h]);
       }
     }
-    gam = wave_sum(gam);
-    del = wave_sum(del);
+    gam = wave_total(gam);
+    del = wave_total(del);
     if (first_cont) {
-      rho = wave_sum(rho);
-      sig = wave_sum(sig);
+      rho = wave_total(rho);
+      sig = wave_total(sig);
     }
     int status = 1;  // 1: continue
     double alpha = 0.0, beta = 0.0;
