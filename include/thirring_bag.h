@@ -32,7 +32,7 @@ extern "C" {
 
 /* globals of fermionbag.c (weak in the library) */
 extern int **field;
-extern double m, mu;
+extern double m, mu, U;  /* U: the coupling, named in the checkpoint file */
 extern double *Dinv;
 extern int n_monomers, n_links, current_sign, max_changes;
 void update_linklists(void);
@@ -46,7 +46,11 @@ int thirring_bag_last_status(void);
 
 /* ---- fermion_matrix.c ------------------------------------------------- */
 double fM_index(int t1, int x1, int t2, int x2, double mu);             /* :28-64     */
-void calc_Dinv(void);                                                   /* :191-351   */
+void calc_Dinv(void);  /* :191-351; the on-disk cache of :206-220,342-346 */
+
+/* ---- fermionbag.c: configuration checkpoint ------------------------------ */
+void write_config(void);                                                /* :120-138   */
+void read_config(char *filename);                                       /* :141-161   */
 
 /* ---- vec_ops.c (argument order out, in as the reference) -------------- */
 void fM(double **chi, double **psi);                                    /* :96-131    */

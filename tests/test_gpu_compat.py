@@ -8,6 +8,7 @@ log|det| and 1e-10 in phase, bag ratios 1e-9 relative; the random streams
 and accept decisions are identical.
 """
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -265,7 +266,12 @@ def test_bag_operator_signatures(bag_lib, oracle):
         C.c_void_p.in_dll(bag_lib, "field").value = None
 
 
-def test_bag_calc_dinv_signature(bag_lib, oracle):
+def test_bag_calc_dinv_signature(bag_lib, oracle, tmp_path, monkeypatch):
+    """calc_Dinv equals the oracle's propagator, writes the reference's
+    cache file (fermion_matrix.c:206-220,342-346: named for the lattice and
+    mu, V*V/2 doubles), and a second call reads that file instead of
+    computing (a changed file shows up in Dinv)."""
+    monkeypatch.chdir(tmp_path)
     nt, nx = 6, 6
     field, B = _bag_setup(bag_lib, oracle, nt, nx, 0, 1, 0.0, 0.1, np.zeros(36, np.int32))
     h = nt * nx // 2
@@ -274,10 +280,66 @@ def test_bag_calc_dinv_signature(bag_lib, oracle):
     g.value = buf.ctypes.data
     try:
         bag_lib.calc_Dinv()
-        assert np.max(np.abs(buf[:2 * h * h] - oracle.calc_Dinv(B))) <= 1e-12
+        ref = oracle.calc_Dinv(B)
+        assert np.max(np.abs(buf[:2 * h * h] - ref)) <= 1e-12
+        cache = tmp_path / "free_propagator_eo_T6X6_mu0.100000"
+        assert cache.exists() and cache.stat().st_size == 8 * 2 * h * h
+        assert np.array_equal(np.fromfile(cache, dtype=np.float64), buf[:2 * h * h])
+        changed = 2.0 * buf[:2 * h * h]
+        changed.tofile(cache)
+        buf[:] = 0.0
+        bag_lib.calc_Dinv()
+        assert np.array_equal(buf[:2 * h * h], changed)
+        assert bag_lib.thirring_bag_last_status() == 0
     finally:
         g.value = None
         C.c_void_p.in_dll(bag_lib, "field").value = None
+
+
+def test_config_checkpoint(bag_lib, oracle, tmp_path, monkeypatch):
+    """write_config / read_config (fermionbag.c:120-161): the file is named
+    for the lattice, U, m and mu and holds NT*NX int32 at t*NX + x; reading
+    it back restores the field."""
+    monkeypatch.chdir(tmp_path)
+    nt, nx = 8, 6
+    rng = np.random.default_rng(31)
+    f = rng.integers(0, 7, size=nt * nx).astype(np.int32)
+    C.c_double.in_dll(bag_lib, "U").value = 0.5
+    field, B = _bag_setup(bag_lib, oracle, nt, nx, 0, 1, 0.1, 0.0, f)
+    try:
+        bag_lib.write_config()
+        path = tmp_path / "config_checkpoint_T8X6_U0.5m0.1mu0"
+        assert path.exists()
+        assert np.array_equal(np.fromfile(path, dtype=np.int32), f)
+        other = Rows(nt, nx, np.int32, np.zeros(nt * nx, np.int32))
+        C.c_void_p.in_dll(bag_lib, "field").value = other.ptr.value
+        bag_lib.read_config(str(path).encode())
+        assert np.array_equal(other.natural(), f)
+        assert bag_lib.thirring_bag_last_status() == 0
+    finally:
+        C.c_void_p.in_dll(bag_lib, "field").value = None
+
+
+def test_read_config_missing_file_exits(tmp_path):
+    """A configuration file that cannot be opened ends the program with the
+    reference's message and status 1 (fermionbag.c:149-155)."""
+    import subprocess
+    import sys
+    code = (
+        "import ctypes as C, numpy as np, sys\n"
+        f"sys.path.insert(0, {os.path.dirname(os.path.dirname(os.path.abspath(__file__)))!r})\n"
+        f"sys.path.insert(0, {os.path.dirname(os.path.abspath(__file__))!r})\n"
+        "from jagged import load, Rows\n"
+        "lib = load('libthirring_bag.so')\n"
+        "assert lib.thirring_bag_init(4, 4, 0, 1) == 0\n"
+        "f = Rows(4, 4, np.int32, np.zeros(16, np.int32))\n"
+        "C.c_void_p.in_dll(lib, 'field').value = f.ptr.value\n"
+        "lib.read_config(b'no_such_configuration')\n"
+        "print('returned')\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 1, r.stdout + r.stderr
+    assert "Could not read configuration" in r.stdout and "returned" not in r.stdout
 
 
 def test_bag_update_sequence(bag_lib, oracle):

@@ -7,8 +7,10 @@
 #include "../../include/thirring_bag.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/time.h>
 
 #include "../../include/th2d.h"
 #include "../../include/th2d_bag.h"
@@ -21,6 +23,7 @@
 __attribute__((weak)) int **field = NULL;
 __attribute__((weak)) double m = 0.0;
 __attribute__((weak)) double mu = 0.0;
+__attribute__((weak)) double U = 0.0;
 __attribute__((weak)) double *Dinv = NULL;
 __attribute__((weak)) int n_monomers = 0;
 __attribute__((weak)) int n_links = 0;
@@ -138,12 +141,97 @@ double fM_index(int t1, int x1, int t2, int x2, double mu_)
   return v;
 }
 
+/* The free propagator is cached on disk as fermion_matrix.c:206-220 and
+ * 342-346 do: a file named for the lattice, mu and the boundary condition
+ * in the working directory holds the V*V/2 doubles of Dinv (its eo block
+ * layout, host byte order).  A file that exists is read instead of
+ * computing; a short file is ignored here (the reference reads it
+ * unchecked).  Otherwise Dinv is computed on the GPU and the file written. */
 void calc_Dinv(void)
 {
   if (!ready()) return;
-  if (fail(th2d_bag_calc_dinv(S.ctx, &S.geo, S.dD))) return;
+  const size_t half = (size_t)S.V * S.V / 2;
   if (!Dinv) Dinv = malloc(sizeof(double) * (size_t)S.V * S.V);
-  if (Dinv) fail(th2d_d2h(S.ctx, Dinv, S.dD, sizeof(double) * (size_t)S.V * S.V / 2));
+  if (!Dinv) {
+    S.status = TH2D_ENOMEM;
+    return;
+  }
+  char filename[100];
+  snprintf(filename, sizeof(filename), S.geo.bc == 1 ? "free_propagator_eo_T%dX%d_mu%0.6f_open"
+                                                      : "free_propagator_eo_T%dX%d_mu%0.6f",
+           S.nt, S.nx, mu);
+  FILE *f = fopen(filename, "rb");
+  if (f) {
+    const size_t got = fread(Dinv, sizeof(double), half, f);
+    fclose(f);
+    if (got == half) return;
+  }
+  struct timeval start, end;
+  gettimeofday(&start, NULL);
+  if (fail(th2d_bag_calc_dinv(S.ctx, &S.geo, S.dD))) return;
+  if (fail(th2d_d2h(S.ctx, Dinv, S.dD, sizeof(double) * half))) return;
+  gettimeofday(&end, NULL);
+  const long diff = 1000000L * (end.tv_sec - start.tv_sec) + (end.tv_usec - start.tv_usec);
+  printf("Inverted fermion matrix in %.3g seconds\n", 1e-6 * diff);
+  f = fopen(filename, "wb");
+  if (f) {
+    fwrite(Dinv, sizeof(double), half, f);
+    fclose(f);
+  }
+}
+
+/* ---- fermionbag.c: configuration checkpoint (:120-161) -------------------
+ * NT*NX int32 values field[t][x] at t*NX + x, host byte order; a file that
+ * cannot be opened ends the program with the reference's message. */
+void write_config(void)
+{
+  if (!S.nt || !field) {
+    S.status = TH2D_EARG;
+    return;
+  }
+  char filename[100];
+  snprintf(filename, sizeof(filename), "config_checkpoint_T%dX%d_U%.6gm%.6gmu%.6g", S.nt, S.nx, U,
+           m, mu);
+  int *buffer = malloc(sizeof(int) * (size_t)S.V);
+  if (!buffer) {
+    S.status = TH2D_ENOMEM;
+    return;
+  }
+  for (int t = 0; t < S.nt; t++)
+    for (int x = 0; x < S.nx; x++) buffer[t * S.nx + x] = field[t][x];
+  FILE *f = fopen(filename, "wb");
+  if (f) {
+    fwrite(buffer, sizeof(int), (size_t)S.V, f);
+    fclose(f);
+  } else {
+    printf("Could not write configuration\n");
+    exit(1);
+  }
+  free(buffer);
+}
+
+void read_config(char *filename)
+{
+  if (!S.nt || !field || !filename) {
+    S.status = TH2D_EARG;
+    return;
+  }
+  int *buffer = calloc((size_t)S.V, sizeof(int));
+  if (!buffer) {
+    S.status = TH2D_ENOMEM;
+    return;
+  }
+  FILE *f = fopen(filename, "rb");
+  if (f) {
+    if (fread(buffer, sizeof(int), (size_t)S.V, f) != (size_t)S.V) S.status = TH2D_EARG;
+    fclose(f);
+  } else {
+    printf("Could not read configuration\n");
+    exit(1);
+  }
+  for (int t = 0; t < S.nt; t++)
+    for (int x = 0; x < S.nx; x++) field[t][x] = buffer[t * S.nx + x];
+  free(buffer);
 }
 
 /* ---- vec_ops.c ---------------------------------------------------------- */
