@@ -693,6 +693,11 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   V2 *R2 = (V2 *)rlds, *Y1 = R2 + E2, *LKt = Y1 + E1, *LKx = LKt + E2;
   double *red = rlds + LDS_D;
   const int tid = threadIdx.x;
+  // a wave holds one row of the region: element q of a thread lies at row
+  // wrow + 8 q (uniform in the wave, kept in a scalar register so that the
+  // row tests branch on the scalar unit) and column lane
+  static_assert(TX == 64, "a wave covers one row of a region");
+  const int wrow = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int tau = ctl.wg_tile[blockIdx.x];  // uniform: idle workgroups leave at once
   if (tau < 0) return;
   const CgTile tl = a.tiles[tau];
@@ -749,7 +754,7 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   // (a region larger than what is left of it) are halo, reloaded each time
   // (byte offset of element q in a chain vector)
   auto oi_of = [&](int q) {
-    const int e = tid + q * NT, lt = e / TX, lx = e - lt * TX;
+    const int lt = wrow + q * (NT / TX), lx = lane;
     return (unsigned)eo_index(G, wrapn(tl.t0 + lt, G.nt), wrapn(tl.x0 + lx, G.nx)) *
            (unsigned)sizeof(V2);
   };
@@ -758,7 +763,7 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   const V2 zero = cmk<V2>((T)0, (T)0);
 #pragma unroll
   for (int q = 0; q < NO; q++) {
-    const int e = tid + q * NT, lt = e / TX, lx = e - lt * TX;
+    const int lt = wrow + q * (NT / TX), lx = lane;
     const unsigned oi = oi_of(q);
     if (lt < tte && lx < txe) {
       inl |= 1u << q;
@@ -773,13 +778,16 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
     }
   }
   unsigned ri[RES_NQ];  // publication offsets of the halo elements (kNoLoad: none)
+  int re[RES_NQ];       // their places in R2
 #pragma unroll
   for (int h = 0; h < RES_NQ; h++) {
     ri[h] = kNoLoad;
+    re[h] = 0;
     if (tid + h * NT < RES_NR) {
       int et, ex;
       res_ring_pos(tid + h * NT, et, ex);
       ri[h] = src_at(et - 2, ex - 2);
+      re[h] = et * W2 + ex;
     }
   }
   // the regions of the chain were placed on one XCD (workgroups are
@@ -811,21 +819,29 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   for (int j = -1;; j++) {
     const int ns = setup ? 0 : slot ^ 1;  // parity written by this pass
     if (!setup) {
-      // 3a. s_j, r_{j+1} on the region and the halo; p_j, x_{j+1} (the
-      // region first: it does not wait for the halo loads)
+      // 3a. s_j, r_{j+1} on the region and the halo; p_j, x_{j+1}.  Elements
+      // outside the lattice part of the region (a region larger than what is
+      // left of the lattice) first take the owners' values, in a pass of its
+      // own: the wait for those loads then comes before this iteration's
+      // stores (a wait for the counter of loads also waits for the stores
+      // issued before it, one store round trip per element).  The region
+      // before the halo ring: it does not wait for the halo loads.
 #pragma unroll
       for (int q = 0; q < NO; q++) {
-        const int e = tid + q * NT, lt = e / TX, lx = e - lt * TX;
-        const bool own = (inl >> q) & 1u;
-        if (!own) {  // outside the lattice part of the region: the owner's values
-          const unsigned so = src_at(lt, lx);
-          r[q] = w[q] = s[q] = zero;
-          if (so != kNoLoad) {
-            r[q] = M.ld2(o_r[slot], so);
-            w[q] = M.ld2(o_w[slot], so);
-            s[q] = M.ld2(o_s[slot], so);
-          }
+        if ((inl >> q) & 1u) continue;
+        const int lt = wrow + q * (NT / TX), lx = lane;
+        const unsigned so = src_at(lt, lx);
+        r[q] = w[q] = s[q] = zero;
+        if (so != kNoLoad) {
+          r[q] = M.ld2(o_r[slot], so);
+          w[q] = M.ld2(o_w[slot], so);
+          s[q] = M.ld2(o_s[slot], so);
         }
+      }
+#pragma unroll
+      for (int q = 0; q < NO; q++) {
+        const int lt = wrow + q * (NT / TX), lx = lane;
+        const bool own = (inl >> q) & 1u;
         const V2 sn = cadd(w[q], cscale(s[q], tb));
         const V2 rn = csub(r[q], cscale(sn, ta));
         R2[(lt + 2) * W2 + (lx + 2)] = rn;
@@ -845,15 +861,13 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       for (int h = 0; h < RES_NQ; h++) {
         if (tid + h * NT >= RES_NR) continue;
         const V2 sn = cadd(hw[h], cscale(hs[h], tb));
-        int et, ex;
-        res_ring_pos(tid + h * NT, et, ex);
-        R2[et * W2 + ex] = csub(hr[h], cscale(sn, ta));
+        R2[re[h]] = csub(hr[h], cscale(sn, ta));
       }
     } else {
 #pragma unroll
       for (int q = 0; q < NO; q++)
         if ((pub >> q) & 1u) {
-          const int e = tid + q * NT, lt = e / TX, lx = e - lt * TX;
+          const int lt = wrow + q * (NT / TX), lx = lane;
           const unsigned po = pub_at(lt, lx);
           M.st2(o_r[0], po, r[q]);
           M.st2(o_s[0], po, s[q]);
@@ -888,7 +902,7 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
 #pragma unroll
     for (int q = 0; q < NO; q++) {
       if (!((inl >> q) & 1u)) continue;
-      const int e = tid + q * NT, lt = e / TX, lx = e - lt * TX;
+      const int lt = wrow + q * (NT / TX), lx = lane;
       const int c1 = (lt + 1) * W1 + (lx + 1);
       const int c2i = (lt + 2) * W2 + (lx + 2);
       const V2 kt = LKt[c2i], kx = LKx[c2i], ktd = LKt[c2i - W2], kxd = LKx[c2i - 1];
