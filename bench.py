@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # (profiles/k_cg_resident_hmc256_16chains_mixed.md), k_cg_step at 32 chains
 # (profiles/k_cg_step_hmc256_32chains_mixed.md)
 # key: (lattice size, chains, resident solver)
-TRAFFIC_BYTES = {(256, 32, False): 199.6e6, (256, 16, True): 112.1e6}
+TRAFFIC_BYTES = {(256, 32, False): 199.6e6, (256, 16, True): 112.8e6}
 
 
 def parse():
