@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--chains", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dslash", action="store_true")
+    ap.add_argument("--dslash-sweep", action="store_true",
+                    help="only the Dslash size sweep 256^2..4096^2 (FP64, FP32; 1 and 8 RHS at 256^2)")
     ap.add_argument("--only-headline", action="store_true",
                     help="time the batched mixed-precision leg only (for profiling)")
     ap.add_argument("--c4-steps", type=int, default=3,
@@ -257,6 +259,25 @@ def dslash_bw(H, N, size, nrhs, precision, reps=50):
     return per, bytes_ / (per * 1e-3) / 1e9
 
 
+def dslash_sweep():
+    """Algorithmic bandwidth of th2d_dslash from 256^2 to 4096^2 (FP64 and
+    FP32, one RHS; 8 RHS at 256^2): the sizes above ~2048^2 (FP64 vectors
+    and links of 200 MB and more) leave the 256 MB last-level cache."""
+    from thirring2d_amd import _native as N
+    from thirring2d_amd import hmc as H
+    out = {}
+    for size in (256, 512, 1024, 2048, 4096):
+        for prec, tag in ((N.PREC_F64, "f64"), (N.PREC_F32, "f32")):
+            for nrhs in ((1, 8) if size == 256 else (1,)):
+                ms, gbs = dslash_bw(H, N, size, nrhs, prec)
+                cb = 16 if prec == N.PREC_F64 else 8
+                out[f"{size}_{tag}_{nrhs}rhs"] = {
+                    "ms": ms, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS,
+                    "bytes": size * size * (2 * cb + nrhs * 2 * cb)}
+    print(json.dumps({"dslash_sweep": out, "unit": "GB/s algorithmic", "peak": HBM_PEAK_GBS}),
+          flush=True)
+
+
 def cpu_baseline(iters_per_traj, size):
     """The oracle's CG iteration at size^2 on one host core, scaled to a
     trajectory by the measured GPU iteration count per trajectory."""
@@ -286,6 +307,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.dslash_sweep:
+        dslash_sweep()
         return
     from thirring2d_amd import _native as N
     from thirring2d_amd import hmc as H
