@@ -8,12 +8,14 @@
 // Every kernel takes the whole batch; matrices of different orders share a
 // launch (blocks of a matrix that is already finished return at once).
 //
-// LU: right-looking, panel width NB = 32.  Per panel k:
-//   panel  -- one block per matrix factors columns k..k+NB-1 with partial
-//             pivoting (whole rows are swapped, as LAPACK's row interchanges
-//             do when applied to the full matrix);
+// LU: right-looking with partial pivoting (the factors equal those of
+// LAPACK's whole-row interchanges).  Up to order PR*PT, blocks of LOB = 128
+// columns, each factored as register panels of 16 columns; then
 //   trsm   -- U12 = L11^{-1} A12 (unit lower, one thread per column);
-//   update -- A22 -= L21 U12, a GEMM on MFMA 16x16x4 tiles (FP64 or FP32).
+//   update -- A22 -= L21 U12 of depth 128, a GEMM on MFMA 16x16x4 tiles
+//             (FP64 or FP32).
+// Larger orders use panels of NB = 32 columns reduced through LDS, with a
+// trailing update per panel.
 #include "th2d_internal.h"
 
 #include <algorithm>
@@ -195,26 +197,60 @@ __global__ void __launch_bounds__(256) k_lu_panel(T *base, BatchView bv, int *pi
 
 // Panel of width PNB held in registers: one 1024-thread block per matrix,
 // row k + t + q*PT of the panel in thread t, slot q < PR (orders up to
-// PR*PT).  Per column: the pivot by a wave argmax and a pass over the 16
-// wave results (largest |a_ij|, ties to the smaller row, as k_lu_panel),
-// the exchange of the two rows' panel parts through LDS, the scaling of the
-// column and the rank-1 update of the panel's later columns, all on
-// registers.  Only the panel's columns are swapped here; k_lu_swap applies
-// the panel's interchanges to the other columns afterwards, in order, so
-// the factors equal those of whole-row swaps.
+// PR*PT).  Per column, with one barrier: each wave finds its largest |a_ij|
+// (a max by DPP moves, then the smallest row attaining it by ballots, so
+// ties go to the smaller row as in k_lu_panel) and posts it with that row's
+// panel part to LDS, and the thread holding the diagonal row posts that
+// row; after the barrier every thread picks the pivot from the 16 posts and
+// does the exchange, the scaling of the column and the rank-1 update of the
+// panel's later columns on registers.  Only the panel's columns are swapped
+// here; k_lu_swap applies the panel's interchanges to the other columns
+// afterwards, in order, so the factors equal those of whole-row swaps.
 constexpr int PNB = 16;   // register panel width
 constexpr int PT = 1024;  // threads of the register panel
 constexpr int PR = 3;     // panel rows per thread
+constexpr int PW = PT / 64;
+constexpr int PCH = 256;  // rows per LDS staging chunk of the panel's load and store
+constexpr int LOB = 128;  // outer block of the register-panel LU
+
+// DPP move of a double across lanes; lanes the move does not write keep v.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_keep(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, ROW_MASK, 0xf,
+                                             false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Wave64 maximum: quads, half-rows, rows, then rows 0..3 into lane 63,
+// which readlane gives to every lane.
+__device__ __forceinline__ double wave_max(double v)
+{
+  v = fmax(v, dpp_keep<0xB1, 0xf>(v));   // quad_perm [1,0,3,2]
+  v = fmax(v, dpp_keep<0x4E, 0xf>(v));   // quad_perm [2,3,0,1]
+  v = fmax(v, dpp_keep<0x141, 0xf>(v));  // row_half_mirror
+  v = fmax(v, dpp_keep<0x140, 0xf>(v));  // row_mirror
+  v = fmax(v, dpp_keep<0x142, 0xa>(v));  // row_bcast:15 into rows 1, 3
+  v = fmax(v, dpp_keep<0x143, 0xc>(v));  // row_bcast:31 into rows 2, 3
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | lo);
+}
 
 template <typename T>
 __global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int *pivb, int *info,
                                                      int k)
 {
   // double-buffered by column parity: a buffer is rewritten two columns
-  // later, after every thread has passed the barriers between
-  __shared__ double svalb[2][PT / 64];
-  __shared__ int sidxb[2][PT / 64];
-  __shared__ T prowb[2][PNB], xrowb[2][PNB];
+  // later, after every thread has passed the barrier between
+  __shared__ double wbest[2][PW];
+  __shared__ int widx[2][PW];
+  __shared__ T wrow[2][PW][PNB];  // each wave's candidate row
+  __shared__ T drow[2][PNB];      // the diagonal row before the exchange
+  __shared__ T stage[PCH * (PNB + 1)];
   const int b = blockIdx.x;
   const int n = bv.n[b];
   if (k >= n) return;
@@ -223,73 +259,91 @@ __global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int 
   const int jb = min(PNB, n - k), tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   T a[PR][PNB];
 #pragma unroll
-  for (int q = 0; q < PR; q++) {
-    const int i = k + tid + q * PT;
+  for (int q = 0; q < PR; q++)
 #pragma unroll
-    for (int c = 0; c < PNB; c++) a[q][c] = (i < n && c < jb) ? A[(long)i * n + k + c] : (T)0;
-  }
+    for (int c = 0; c < PNB; c++) a[q][c] = (T)0;
+  // rows arrive through LDS in chunks of PCH, read row-contiguously (16
+  // lanes per row) rather than one row per lane
+#pragma unroll
+  for (int q = 0; q < PR; q++)
+#pragma unroll
+    for (int s = 0; s < PT / PCH; s++) {
+      const int r0 = q * PT + s * PCH;  // first panel row of the chunk, from k
+      if (k + r0 >= n) continue;
+      __syncthreads();
+      for (int e = tid; e < PCH * PNB; e += PT) {
+        const int r = e / PNB, c = e - r * PNB, i = k + r0 + r;
+        stage[r * (PNB + 1) + c] = (i < n && c < jb) ? A[(long)i * n + k + c] : (T)0;
+      }
+      __syncthreads();
+      const int r = tid - s * PCH;
+      if (r >= 0 && r < PCH)
+#pragma unroll
+        for (int c = 0; c < PNB; c++) a[q][c] = stage[r * (PNB + 1) + c];
+    }
 #pragma clang loop unroll(full)
   for (int j = 0; j < PNB; j++) {
     if (j >= jb) continue;  // (a full unroll keeps a[][] in registers)
-    const int gj = k + j;
-    double *sval = svalb[j & 1];
-    int *sidx = sidxb[j & 1];
-    T *prow = prowb[j & 1], *xrow = xrowb[j & 1];
+    const int gj = k + j, par = j & 1;
+    // the thread's largest |a_ij| over rows i >= gj, first row on ties
     double best = -1.0;
-    int bi = n;
+    int bq = 0;
 #pragma unroll
     for (int q = 0; q < PR; q++) {
       const int i = k + tid + q * PT;
       const double v = fabs((double)a[q][j]);
-      if (i >= gj && i < n && v > best) {  // rows of a thread ascend: ties keep the first
+      if (i >= gj && i < n && v > best) {
         best = v;
-        bi = i;
+        bq = q;
       }
     }
+    // the wave's: rows ascend with (q, lane), so the first q whose ballot
+    // of lanes at the maximum is not empty, and its lowest lane
+    const double wm = wave_max(best);
+    int qs = -1, ls = 0;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const double v2 = __shfl_xor(best, off, 64);
-      const int i2 = __shfl_xor(bi, off, 64);
-      if (v2 > best || (v2 == best && i2 < bi)) {
-        best = v2;
-        bi = i2;
+    for (int q = PR - 1; q >= 0; q--) {
+      const unsigned long long m = __ballot(best == wm && bq == q);
+      if (m) {
+        qs = q;
+        ls = __builtin_ctzll(m);
       }
     }
+    if (wm < 0.0) qs = -1;  // no rows of this wave are candidates
+#pragma unroll
+    for (int q = 0; q < PR; q++)
+      if (q == qs && lane == ls) {
+#pragma unroll
+        for (int c = 0; c < PNB; c++) wrow[par][wv][c] = a[q][c];
+      }
     if (lane == 0) {
-      sval[wv] = best;
-      sidx[wv] = bi;
+      wbest[par][wv] = qs < 0 ? -1.0 : wm;
+      widx[par][wv] = qs < 0 ? n : k + wv * 64 + ls + qs * PT;
+    }
+    if (tid == j) {  // row gj = k + j is slot 0 of thread j
+#pragma unroll
+      for (int c = 0; c < PNB; c++) drow[par][c] = a[0][c];
     }
     __syncthreads();
-    best = sval[0];
-    bi = sidx[0];
-    for (int w = 1; w < PT / 64; w++) {
-      const double v2 = sval[w];
-      const int i2 = sidx[w];
-      if (v2 > best || (v2 == best && i2 < bi)) {
-        best = v2;
+    double bv2 = wbest[par][0];
+    int bi = widx[par][0], bw = 0;
+#pragma unroll
+    for (int w = 1; w < PW; w++) {
+      const double v2 = wbest[par][w];
+      const int i2 = widx[par][w];
+      if (v2 > bv2 || (v2 == bv2 && i2 < bi)) {
+        bv2 = v2;
         bi = i2;
+        bw = w;
       }
     }
     const int p = bi;  // >= gj: row gj itself is a candidate
     if (tid == 0) {
       piv[gj] = p;
-      if (best == 0.0 && info[b] == 0) info[b] = gj + 1;
+      if (bv2 == 0.0 && info[b] == 0) info[b] = gj + 1;
     }
-    // exchange rows gj and p of the panel: the pivot row to prow, the
-    // displaced row to xrow
-#pragma unroll
-    for (int q = 0; q < PR; q++) {
-      const int i = k + tid + q * PT;
-      if (i == p) {
-#pragma unroll
-        for (int c = 0; c < PNB; c++) prow[c] = a[q][c];
-      }
-      if (i == gj) {
-#pragma unroll
-        for (int c = 0; c < PNB; c++) xrow[c] = a[q][c];
-      }
-    }
-    __syncthreads();
+    const T *prow = wrow[par][bw], *xrow = drow[par];
+    // exchange rows gj and p of the panel
 #pragma unroll
     for (int q = 0; q < PR; q++) {
       const int i = k + tid + q * PT;
@@ -317,13 +371,22 @@ __global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int 
     }
   }
 #pragma unroll
-  for (int q = 0; q < PR; q++) {
-    const int i = k + tid + q * PT;
-    if (i < n)
+  for (int q = 0; q < PR; q++)
 #pragma unroll
-      for (int c = 0; c < PNB; c++)
-        if (c < jb) A[(long)i * n + k + c] = a[q][c];
-  }
+    for (int s = 0; s < PT / PCH; s++) {
+      const int r0 = q * PT + s * PCH;
+      if (k + r0 >= n) continue;
+      __syncthreads();
+      const int r = tid - s * PCH;
+      if (r >= 0 && r < PCH)
+#pragma unroll
+        for (int c = 0; c < PNB; c++) stage[r * (PNB + 1) + c] = a[q][c];
+      __syncthreads();
+      for (int e = tid; e < PCH * PNB; e += PT) {
+        const int r2 = e / PNB, c = e - r2 * PNB, i = k + r0 + r2;
+        if (i < n && c < jb) A[(long)i * n + k + c] = stage[r2 * (PNB + 1) + c];
+      }
+    }
 }
 
 // The row interchanges of the panel at k (pivots k..k+jb-1, in order) on
@@ -382,26 +445,24 @@ __global__ void __launch_bounds__(256) k_lu_trsm(T *base, BatchView bv, int k, i
   }
 }
 
-// A22 -= L21 U12 for the trailing block after panel k.
-// (columns k + jb .. c1 - 1 only when c1 > 0)
+// A[r0:r1, c0:c1] -= A[r0:r1, kk:kk+K] A[kk:kk+K, c0:c1], every range
+// clipped to the matrix's order: the trailing and in-block updates of the
+// LU (L21 U12).
 template <typename T, bool MF>
-__global__ void __launch_bounds__(256) k_lu_update(T *base, BatchView bv, int k, int nb, int c1)
+__global__ void __launch_bounds__(256) k_lu_update(T *base, BatchView bv, int kk, int K, int r0,
+                                                   int r1, int c0, int c1)
 {
   __shared__ T sA[BM * (BK + 1)];
   __shared__ T sB[BK * (BN + 1)];
   const int b = blockIdx.z;
   const int n = bv.n[b];
-  if (k >= n) return;
-  const int jb = min(nb, n - k);
-  const int m = n - k - jb;                                 // rows
-  const int mc = (c1 > 0 ? min(c1, n) : n) - (k + jb);      // columns
-  if (m <= 0 || mc <= 0) return;
+  const int kn = min(K, n - kk), m = min(r1, n) - r0, mc = min(c1, n) - c0;
+  if (kn <= 0 || m <= 0 || mc <= 0) return;
   const int ti = blockIdx.y * BM, tj = blockIdx.x * BN;
   if (ti >= m || tj >= mc) return;
   T *A = base + bv.off[b];
-  const long o = (long)(k + jb) * n + (k + jb);
-  gemm_tile<T, MF>(A + (long)(k + jb) * n + k, n, A + (long)k * n + (k + jb), n, A + o, n, m, mc,
-                   jb, (T)-1, ti, tj, sA, sB);
+  gemm_tile<T, MF>(A + (long)r0 * n + kk, n, A + (long)kk * n + c0, n, A + (long)r0 * n + c0, n,
+                   m, mc, kn, (T)-1, ti, tj, sA, sB);
 }
 
 // log|det| and sign from the factors: sign = prod sign(u_ii) * (-1)^{#swaps}
@@ -565,49 +626,67 @@ bool use_mfma()
 BatchView view(const th2d_batch *b) { return BatchView{b->off, b->n, b->poff, b->count}; }
 
 template <typename T>
+void launch_update(th2d_ctx *c, bool mf, T *A, BatchView bv, int count, int kk, int K, int r0,
+                   int r1, int c0, int c1, int N)
+{
+  const int rows = std::min(r1, N) - r0, cols = std::min(c1, N) - c0;
+  if (rows <= 0 || cols <= 0) return;
+  dim3 g((cols + BN - 1) / BN, (rows + BM - 1) / BM, count);
+  if (mf)
+    hipLaunchKernelGGL((k_lu_update<T, true>), g, dim3(256), 0, c->stream, A, bv, kk, K, r0, r1,
+                       c0, c1);
+  else
+    hipLaunchKernelGGL((k_lu_update<T, false>), g, dim3(256), 0, c->stream, A, bv, kk, K, r0, r1,
+                       c0, c1);
+}
+
+template <typename T>
 int lu_run(th2d_ctx *c, const th2d_batch *bt, T *A, int *piv, int *info)
 {
   BatchView bv = view(bt);
   TH2D_TRY(th2d_memset(c, info, 0, sizeof(int) * bt->count));
   const bool mf = use_mfma();
-  // panels in registers when every matrix's rows fit the block (PR rows
-  // per thread): blocks of OB = 2 PNB columns, each factored as two
-  // register panels (the second after the first's update of its columns),
-  // then one trailing update of width OB; else the LDS-reduction panel of
-  // width NB
-  const bool reg = bt->max_n <= PR * PT && g_lu_reg_panel;
-  constexpr int OB = 2 * PNB;
-  static_assert(OB == NB, "the outer block is the LDS panel's width");
-  const int N = bt->max_n;
-  for (int k = 0; k < N; k += NB) {
-    if (reg) {
-      for (int kk = k; kk < std::min(k + OB, N); kk += PNB) {
-        hipLaunchKernelGGL(k_lu_panel_reg<T>, dim3(bt->count), dim3(PT), 0, c->stream, A, bv, piv,
+  const int N = bt->max_n, cnt = bt->count;
+  constexpr int END = 1 << 30;  // a range bound past every order
+  if (bt->max_n <= PR * PT && g_lu_reg_panel) {
+    // blocks of LOB columns.  Inside a block: register panels of PNB
+    // columns, each followed by its row interchanges on the other columns,
+    // U of its rows for the block's later columns and their update.  After
+    // the block: U12 of the block's rows for every later column, by
+    // triangular solves on NB-row slices with the updates between them,
+    // then one trailing update A22 -= L21 U12 of depth LOB.
+    for (int k = 0; k < N; k += LOB) {
+      const int ce = std::min(k + LOB, N);
+      for (int kk = k; kk < ce; kk += PNB) {
+        hipLaunchKernelGGL(k_lu_panel_reg<T>, dim3(cnt), dim3(PT), 0, c->stream, A, bv, piv,
                            info, kk);
-        hipLaunchKernelGGL(k_lu_swap<T>, dim3((N + 255) / 256, bt->count), dim3(256), 0,
-                           c->stream, A, bv, piv, kk, PNB);
-        const int cb = kk + PNB, ce = std::min(k + OB, N);  // the block's later columns
+        hipLaunchKernelGGL(k_lu_swap<T>, dim3((N + 255) / 256, cnt), dim3(256), 0, c->stream, A,
+                           bv, piv, kk, PNB);
+        const int cb = kk + PNB;
         if (cb < ce) {
-          hipLaunchKernelGGL((k_lu_trsm<T, PNB>), dim3(1, bt->count), dim3(256), 0, c->stream, A,
-                             bv, kk, cb, ce);
-          dim3 g((ce - cb + BN - 1) / BN, (N - cb + BM - 1) / BM, bt->count);
-          if (mf)
-            hipLaunchKernelGGL((k_lu_update<T, true>), g, dim3(256), 0, c->stream, A, bv, kk, PNB, ce);
-          else
-            hipLaunchKernelGGL((k_lu_update<T, false>), g, dim3(256), 0, c->stream, A, bv, kk, PNB, ce);
+          hipLaunchKernelGGL((k_lu_trsm<T, PNB>), dim3(1, cnt), dim3(256), 0, c->stream, A, bv,
+                             kk, cb, ce);
+          launch_update<T>(c, mf, A, bv, cnt, kk, PNB, cb, END, cb, ce, N);
         }
       }
-    } else {
-      hipLaunchKernelGGL(k_lu_panel<T>, dim3(bt->count), dim3(256), 0, c->stream, A, bv, piv,
-                         info, k);
+      if (ce >= N) break;
+      for (int s = k; s < ce; s += NB) {
+        hipLaunchKernelGGL((k_lu_trsm<T, NB>), dim3((N - ce + 255) / 256, cnt), dim3(256), 0,
+                           c->stream, A, bv, s, ce, END);
+        if (s + NB < ce) launch_update<T>(c, mf, A, bv, cnt, s, NB, s + NB, ce, ce, END, N);
+      }
+      launch_update<T>(c, mf, A, bv, cnt, k, LOB, ce, END, ce, END, N);
     }
-    const int rest = N - k - NB;
-    if (rest > 0) {
-      hipLaunchKernelGGL((k_lu_trsm<T, NB>), dim3((rest + 255) / 256, bt->count), dim3(256), 0,
-                         c->stream, A, bv, k, 0, 0);
-      dim3 g((rest + BN - 1) / BN, (rest + BM - 1) / BM, bt->count);
-      if (mf) hipLaunchKernelGGL((k_lu_update<T, true>), g, dim3(256), 0, c->stream, A, bv, k, NB, 0);
-      else hipLaunchKernelGGL((k_lu_update<T, false>), g, dim3(256), 0, c->stream, A, bv, k, NB, 0);
+  } else {
+    // the LDS-reduction panel of width NB, which swaps whole rows itself
+    for (int k = 0; k < N; k += NB) {
+      hipLaunchKernelGGL(k_lu_panel<T>, dim3(cnt), dim3(256), 0, c->stream, A, bv, piv, info, k);
+      const int rest = N - k - NB;
+      if (rest > 0) {
+        hipLaunchKernelGGL((k_lu_trsm<T, NB>), dim3((rest + 255) / 256, cnt), dim3(256), 0,
+                           c->stream, A, bv, k, 0, 0);
+        launch_update<T>(c, mf, A, bv, cnt, k, NB, k + NB, END, k + NB, END, N);
+      }
     }
   }
   TH2D_HIP(hipGetLastError());
