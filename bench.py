@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--no-dslash", action="store_true")
     ap.add_argument("--dslash-sweep", action="store_true",
                     help="only the Dslash size sweep 256^2..4096^2 (FP64, FP32; 1 and 8 RHS at 256^2)")
+    ap.add_argument("--dense-only", action="store_true",
+                    help="only the dense leg: batched LU of 64 bag matrices of order 256..2304 "
+                         "(FP64, FP32), then --c5-proposals bag proposals")
     ap.add_argument("--only-headline", action="store_true",
                     help="time the batched mixed-precision leg only (for profiling)")
     ap.add_argument("--c4-steps", type=int, default=3,
@@ -310,6 +313,10 @@ def main():
         return
     if args.dslash_sweep:
         dslash_sweep()
+        return
+    if args.dense_only:
+        from thirring2d_amd import _native as N
+        print(json.dumps({"c5_bag_dense": c5_leg(N, args.c5_proposals)}), flush=True)
         return
     from thirring2d_amd import _native as N
     from thirring2d_amd import hmc as H
