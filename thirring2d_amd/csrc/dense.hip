@@ -76,19 +76,39 @@ __device__ void gemm_tile(const T *A, long lda, const T *B, long ldb, T *C, long
     for (int a = 0; a < 4; a++)
       for (int b = 0; b < 4; b++) vacc[a][b] = 0;
   }
+  // software pipeline: the next chunk's A and B entries are loaded into
+  // registers while the current chunk is multiplied out of LDS
+  constexpr int EA = BM * BK / 256, EB = BK * BN / 256;
+  T ra[EA], rb[EB];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < EA; q++) {
+      const int e = tid + q * 256, r = e / BK, c = e - r * BK;
+      const int gi = i0 + r, gk = k0 + c;
+      ra[q] = (gi < M && gk < K) ? A[gi * lda + gk] : (T)0;
+    }
+#pragma unroll
+    for (int q = 0; q < EB; q++) {
+      const int e = tid + q * 256, r = e / BN, c = e - r * BN;
+      const int gk = k0 + r, gj = j0 + c;
+      rb[q] = (gk < K && gj < N) ? B[gk * ldb + gj] : (T)0;
+    }
+  };
+  fetch(0);
   for (int k0 = 0; k0 < K; k0 += BK) {
     __syncthreads();
-    for (int e = tid; e < BM * BK; e += 256) {
-      int r = e / BK, c = e - r * BK;
-      int gi = i0 + r, gk = k0 + c;
-      sA[r * (BK + 1) + c] = (gi < M && gk < K) ? A[gi * lda + gk] : (T)0;
+#pragma unroll
+    for (int q = 0; q < EA; q++) {
+      const int e = tid + q * 256, r = e / BK, c = e - r * BK;
+      sA[r * (BK + 1) + c] = ra[q];
     }
-    for (int e = tid; e < BK * BN; e += 256) {
-      int r = e / BN, c = e - r * BN;
-      int gk = k0 + r, gj = j0 + c;
-      sB[r * (BN + 1) + c] = (gk < K && gj < N) ? B[gk * ldb + gj] : (T)0;
+#pragma unroll
+    for (int q = 0; q < EB; q++) {
+      const int e = tid + q * 256, r = e / BN, c = e - r * BN;
+      sB[r * (BN + 1) + c] = rb[q];
     }
     __syncthreads();
+    if (k0 + BK < K) fetch(k0 + BK);
     if (USE_MFMA) {
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
