@@ -215,9 +215,9 @@ __global__ void __launch_bounds__(256) k_lu_panel(T *base, BatchView bv, int *pi
   }
 }
 
-// Panel of width PNB held in registers: one 1024-thread block per matrix,
-// row k + t + q*PT of the panel in thread t, slot q < PR (orders up to
-// PR*PT).  Per column, with one barrier: each wave finds its largest |a_ij|
+// Panel of width PNB held in registers: one block of NT threads per matrix
+// (NT = 256, 512 or 1024, the fewest that hold the rows left), row
+// k + t + q*NT of the panel in thread t, slot q < PR (orders up to PR*PT).  Per column, with one barrier: each wave finds its largest |a_ij|
 // (a max by DPP moves, then the smallest row attaining it by ballots, so
 // ties go to the smaller row as in k_lu_panel) and posts it with that row's
 // panel part to LDS, and the thread holding the diagonal row posts that
@@ -229,7 +229,6 @@ __global__ void __launch_bounds__(256) k_lu_panel(T *base, BatchView bv, int *pi
 constexpr int PNB = 16;   // register panel width
 constexpr int PT = 1024;  // threads of the register panel
 constexpr int PR = 3;     // panel rows per thread
-constexpr int PW = PT / 64;
 constexpr int PCH = 256;  // rows per LDS staging chunk of the panel's load and store
 constexpr int LOB = 128;  // outer block of the register-panel LU
 
@@ -260,15 +259,15 @@ __device__ __forceinline__ double wave_max(double v)
   return __longlong_as_double(((long long)hi << 32) | lo);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int *pivb, int *info,
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_lu_panel_reg(T *base, BatchView bv, int *pivb, int *info,
                                                      int k)
 {
   // double-buffered by column parity: a buffer is rewritten two columns
   // later, after every thread has passed the barrier between
-  __shared__ double wbest[2][PW];
-  __shared__ int widx[2][PW];
-  __shared__ T wrow[2][PW][PNB];  // each wave's candidate row
+  __shared__ double wbest[2][NT / 64];
+  __shared__ int widx[2][NT / 64];
+  __shared__ T wrow[2][NT / 64][PNB];  // each wave's candidate row
   __shared__ T drow[2][PNB];      // the diagonal row before the exchange
   __shared__ T stage[PCH * (PNB + 1)];
   const int b = blockIdx.x;
@@ -287,11 +286,11 @@ __global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int 
 #pragma unroll
   for (int q = 0; q < PR; q++)
 #pragma unroll
-    for (int s = 0; s < PT / PCH; s++) {
-      const int r0 = q * PT + s * PCH;  // first panel row of the chunk, from k
+    for (int s = 0; s < NT / PCH; s++) {
+      const int r0 = q * NT + s * PCH;  // first panel row of the chunk, from k
       if (k + r0 >= n) continue;
       __syncthreads();
-      for (int e = tid; e < PCH * PNB; e += PT) {
+      for (int e = tid; e < PCH * PNB; e += NT) {
         const int r = e / PNB, c = e - r * PNB, i = k + r0 + r;
         stage[r * (PNB + 1) + c] = (i < n && c < jb) ? A[(long)i * n + k + c] : (T)0;
       }
@@ -310,7 +309,7 @@ __global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int 
     int bq = 0;
 #pragma unroll
     for (int q = 0; q < PR; q++) {
-      const int i = k + tid + q * PT;
+      const int i = k + tid + q * NT;
       const double v = fabs((double)a[q][j]);
       if (i >= gj && i < n && v > best) {
         best = v;
@@ -338,7 +337,7 @@ __global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int 
       }
     if (lane == 0) {
       wbest[par][wv] = qs < 0 ? -1.0 : wm;
-      widx[par][wv] = qs < 0 ? n : k + wv * 64 + ls + qs * PT;
+      widx[par][wv] = qs < 0 ? n : k + wv * 64 + ls + qs * NT;
     }
     if (tid == j) {  // row gj = k + j is slot 0 of thread j
 #pragma unroll
@@ -348,7 +347,7 @@ __global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int 
     double bv2 = wbest[par][0];
     int bi = widx[par][0], bw = 0;
 #pragma unroll
-    for (int w = 1; w < PW; w++) {
+    for (int w = 1; w < NT / 64; w++) {
       const double v2 = wbest[par][w];
       const int i2 = widx[par][w];
       if (v2 > bv2 || (v2 == bv2 && i2 < bi)) {
@@ -366,7 +365,7 @@ __global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int 
     // exchange rows gj and p of the panel
 #pragma unroll
     for (int q = 0; q < PR; q++) {
-      const int i = k + tid + q * PT;
+      const int i = k + tid + q * NT;
       if (i == gj) {
 #pragma unroll
         for (int c = 0; c < PNB; c++) a[q][c] = prow[c];
@@ -379,7 +378,7 @@ __global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int 
     const T pv = prow[j];
 #pragma unroll
     for (int q = 0; q < PR; q++) {
-      const int i = k + tid + q * PT;
+      const int i = k + tid + q * NT;
       if (i > gj && i < n) {
         T l = a[q][j];
         if (pv != (T)0) l /= pv;
@@ -393,8 +392,8 @@ __global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int 
 #pragma unroll
   for (int q = 0; q < PR; q++)
 #pragma unroll
-    for (int s = 0; s < PT / PCH; s++) {
-      const int r0 = q * PT + s * PCH;
+    for (int s = 0; s < NT / PCH; s++) {
+      const int r0 = q * NT + s * PCH;
       if (k + r0 >= n) continue;
       __syncthreads();
       const int r = tid - s * PCH;
@@ -402,7 +401,7 @@ __global__ void __launch_bounds__(PT) k_lu_panel_reg(T *base, BatchView bv, int 
 #pragma unroll
         for (int c = 0; c < PNB; c++) stage[r * (PNB + 1) + c] = a[q][c];
       __syncthreads();
-      for (int e = tid; e < PCH * PNB; e += PT) {
+      for (int e = tid; e < PCH * PNB; e += NT) {
         const int r2 = e / PNB, c = e - r2 * PNB, i = k + r0 + r2;
         if (i < n && c < jb) A[(long)i * n + k + c] = stage[r2 * (PNB + 1) + c];
       }
@@ -678,8 +677,16 @@ int lu_run(th2d_ctx *c, const th2d_batch *bt, T *A, int *piv, int *info)
     for (int k = 0; k < N; k += LOB) {
       const int ce = std::min(k + LOB, N);
       for (int kk = k; kk < ce; kk += PNB) {
-        hipLaunchKernelGGL(k_lu_panel_reg<T>, dim3(cnt), dim3(PT), 0, c->stream, A, bv, piv,
-                           info, kk);
+        const int rows = N - kk;  // the most rows a matrix has left
+        if (rows <= PR * 256)
+          hipLaunchKernelGGL((k_lu_panel_reg<T, 256>), dim3(cnt), dim3(256), 0, c->stream, A, bv,
+                             piv, info, kk);
+        else if (rows <= PR * 512)
+          hipLaunchKernelGGL((k_lu_panel_reg<T, 512>), dim3(cnt), dim3(512), 0, c->stream, A, bv,
+                             piv, info, kk);
+        else
+          hipLaunchKernelGGL((k_lu_panel_reg<T, PT>), dim3(cnt), dim3(PT), 0, c->stream, A, bv,
+                             piv, info, kk);
         hipLaunchKernelGGL(k_lu_swap<T>, dim3((N + 255) / 256, cnt), dim3(256), 0, c->stream, A,
                            bv, piv, kk, PNB);
         const int cb = kk + PNB;
