@@ -148,6 +148,32 @@ def test_lu_panels_match_numpy(bag, native, reg):
         assert abs(l - l_np) <= 1e-10 * max(1.0, abs(l_np))
 
 
+@pytest.mark.parametrize("reg", [1, 0])
+def test_lu_factors_match_lapack(bag, native, reg):
+    """The pivots equal LAPACK's partial-pivot choices (scipy's getrf: the
+    first row of largest |a|), and L and U agree with LAPACK's to rounding,
+    for both panels, on orders either side of the 16- and 128-column block
+    edges and of the panel's thread counts, with a matrix of entries in
+    {-1/2, 0, 1/2, 1} full of exact ties."""
+    sl = pytest.importorskip("scipy.linalg")
+    rng = np.random.default_rng(23)
+    ns = [1, 5, 16, 17, 127, 128, 129, 300, 769, 1537, 2304]
+    mats = _mats(rng, ns) + [rng.choice([-0.5, 0.0, 0.5, 1.0], size=(500, 500))]
+    native.lib().th2d_set_lu_panel(reg)
+    try:
+        db = bag.DenseBatch(mats, native.PREC_F64).lu()
+    finally:
+        native.lib().th2d_set_lu_panel(1)
+    piv = db.piv.download((sum(db.ns),), np.int32)
+    o = 0
+    for M, F in zip(mats, db.matrices()):
+        n = M.shape[0]
+        lu, pv = sl.lu_factor(M)
+        assert np.array_equal(piv[o:o + n], pv), n
+        assert np.max(np.abs(F - lu)) <= 1e-11 * np.max(np.abs(lu)), n
+        o += n
+
+
 def test_lu_logdet_f32_bound(bag, native):
     """FP32-MFMA LU: log|det| within the stated bound of the FP64 value."""
     rng = np.random.default_rng(13)
