@@ -67,6 +67,9 @@ def parse():
 
 
 def copy_calibration(mib):
+    """Five device copies of MIB MiB (the scale of the FETCH_SIZE and
+    WRITE_SIZE counters), then twenty timed with HIP events: the read+write
+    rate a plain one-pass stream reaches on this GPU."""
     import torch
     n = mib * (1 << 20) // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -74,7 +77,16 @@ def copy_calibration(mib):
     for _ in range(5):
         dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 5}), flush=True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 20
+    print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 25, "timed_launches": 20,
+                      "ms_per_copy": ms, "GB/s_read_plus_write": 2 * n * 4 / ms / 1e6}),
+          flush=True)
 
 
 def run_batch(H, N, lats, gs, precision, steps, warmup, nsteps=10):
