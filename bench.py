@@ -66,10 +66,10 @@ def parse():
     return ap.parse_args()
 
 
-def copy_calibration(mib):
-    """Five device copies of MIB MiB (the scale of the FETCH_SIZE and
-    WRITE_SIZE counters), then twenty timed with HIP events: the read+write
-    rate a plain one-pass stream reaches on this GPU."""
+def copy_rate(mib):
+    """Five device copies of MIB MiB, then twenty timed with HIP events:
+    (ms per copy, read+write GB/s), the rate a plain one-pass stream reaches
+    on this GPU."""
     import torch
     n = mib * (1 << 20) // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -84,8 +84,17 @@ def copy_calibration(mib):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / 20
-    print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 25, "timed_launches": 20,
-                      "ms_per_copy": ms, "GB/s_read_plus_write": 2 * n * 4 / ms / 1e6}),
+    del src, dst
+    torch.cuda.empty_cache()
+    return ms, 2 * n * 4 / ms / 1e6
+
+
+def copy_calibration(mib):
+    """The copies of copy_rate: the scale of the FETCH_SIZE and WRITE_SIZE
+    counters, and the copy rate."""
+    ms, gbs = copy_rate(mib)
+    print(json.dumps({"copy_calibration_bytes": mib * (1 << 20), "launches": 25,
+                      "timed_launches": 20, "ms_per_copy": ms, "GB/s_read_plus_write": gbs}),
           flush=True)
 
 
