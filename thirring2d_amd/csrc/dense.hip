@@ -434,13 +434,12 @@ __global__ void __launch_bounds__(256) k_lu_swap(T *base, BatchView bv, const in
 
 // U12 = L11^{-1} A12 on rows k..k+jb-1 of the columns c0 <= c < c1 (all
 // from k + jb when c1 <= 0), unit lower L11 of order jb <= JB.  One thread
-// per column; its column of U12 lives in LDS (sx[r][thread]), so the
-// recurrence needs neither unrolling nor registers per row.
+// per column; its column of U12 lives in registers (the loops over the JB
+// rows unroll fully, so every index is a constant), L11 in LDS.
 template <typename T, int JB>
 __global__ void __launch_bounds__(256) k_lu_trsm(T *base, BatchView bv, int k, int c0, int c1)
 {
   __shared__ T sL[JB * JB];
-  __shared__ T sx[JB * 256];
   const int b = blockIdx.y;
   const int n = bv.n[b];
   if (k >= n) return;
@@ -453,14 +452,20 @@ __global__ void __launch_bounds__(256) k_lu_trsm(T *base, BatchView bv, int k, i
     sL[r * JB + q] = A[(long)(k + r) * n + k + q];
   }
   __syncthreads();
-  const int t = threadIdx.x, c = cb + blockIdx.x * 256 + t;
+  const int c = cb + blockIdx.x * 256 + threadIdx.x;
   if (c >= ce) return;
-  for (int r = 0; r < jb; r++) sx[r * 256 + t] = A[(long)(k + r) * n + c];
-  for (int r = 1; r < jb; r++) {
-    T s = sx[r * 256 + t];
-    for (int q = 0; q < r; q++) s -= sL[r * JB + q] * sx[q * 256 + t];
-    sx[r * 256 + t] = s;
-    A[(long)(k + r) * n + c] = s;
+  T x[JB];
+#pragma unroll
+  for (int r = 0; r < JB; r++) x[r] = r < jb ? A[(long)(k + r) * n + c] : (T)0;
+#pragma unroll
+  for (int r = 1; r < JB; r++) {
+    if (r < jb) {
+      T s = x[r];
+#pragma unroll
+      for (int q = 0; q < r; q++) s -= sL[r * JB + q] * x[q];
+      x[r] = s;
+      A[(long)(k + r) * n + c] = s;
+    }
   }
 }
 
