@@ -327,7 +327,7 @@ def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
-              "(independent chains shard trivially, see DESIGN.md)", file=sys.stderr)
+              "(independent chains and bag matrices shard across GPUs without communication)", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

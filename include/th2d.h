@@ -232,7 +232,7 @@ typedef struct {
   int nf;            /* Nf of the gauge action (hmc.c:28)                */
   int nsteps;        /* leapfrog steps (hmc.c:708)                       */
   double traj_len;   /* trajectory length (hmc.c:709)                     */
-  int phys_norm;     /* 0: reference noise normalisation (D4/D5); 1: fixed */
+  int phys_norm;     /* 0: reference noise normalisation (Re, Im ~ N(0,1) each, hmc.c:418-436); 1: scaled by 1/sqrt(2) */
   th2d_cg_params cg; /* solver used for every inversion                 */
 } th2d_hmc_params;
 

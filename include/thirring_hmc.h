@@ -19,8 +19,8 @@
  * x, as hmc.c:143-180.
  *
  * Differences from the reference, by design:
- *  - fm_conjugate_mul is the true adjoint M^dag (the reference body repeats
- *    fm_mul, defect D1 of the survey);
+ *  - fm_conjugate_mul is the true adjoint M^dag (the reference body,
+ *    hmc.c:188-249, repeats fm_mul);
  *  - the solver stop is configurable (thirring_hmc_set_cg); the default is
  *    the reference's absolute r.r < 1e-30 with at most 100000 iterations
  *    (hmc.c:34-35);

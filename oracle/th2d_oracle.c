@@ -386,8 +386,8 @@ int orc_momentum_step(const orc_lattice *L, double *mom, const double *A, const 
     }
 
   /* stochastic part (hmc.c:608-661); identically zero at mu = 0.  The
-   * x branch pairs st(x) with the t neighbour, as the reference does
-   * (defect D6); its two terms cancel for every input. */
+   * x branch pairs st(x) with the t neighbour, as the reference does (an
+   * index slip there); its two terms cancel for every input. */
   for (int t = 0; t < nt; t++)
     for (int x = 0; x < nx; x++) {
       int s = t * nx + x;

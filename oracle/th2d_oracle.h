@@ -29,7 +29,7 @@ extern "C" {
 typedef struct {
   int nt, nx;        /* lattice extent                                  */
   int anti_t;        /* 1: antiperiodic in t (hmc.c:143-158)            */
-  int anti_x;        /* 1: antiperiodic in x (hmc.c:165-180, D9)        */
+  int anti_x;        /* 1: antiperiodic in x (hmc.c:165-180)        */
   double m, g, mu;   /* mass, coupling, chemical potential              */
   int nf;            /* Nf in the gauge action (hmc.c:28)               */
 } orc_lattice;
@@ -61,8 +61,7 @@ int orc_eta(int t, int x, int dir);
 void orc_fm_mul(const orc_lattice *L, const double *A, const double *in, double *out);
 
 /* chi = M^dagger psi, the true conjugate transpose that fm_conjugate_mul
- * (hmc.c:188-249) is meant to be (defect D1: the reference body equals
- * fm_mul).  Parity pinned by: <w,Mv> = <M^dag w, v> and M^dag = 2m - M at
+ * (hmc.c:188-249) is meant to be (the reference body equals fm_mul).  Parity pinned by: <w,Mv> = <M^dag w, v> and M^dag = 2m - M at
  * mu = 0. */
 void orc_fm_dag_mul(const orc_lattice *L, const double *A, const double *in, double *out);
 
@@ -70,7 +69,7 @@ void orc_fm_dag_mul(const orc_lattice *L, const double *A, const double *in, dou
 void orc_fermion_matrix(const orc_lattice *L, const double *A, double *M);
 
 /* Complex partial-pivot LU; log|det| and arg(det) (hmc.c:314-336, but
- * accumulated as logs, defect D21).  Returns 0, or k>0 for a zero pivot.
+ * accumulated as logs, so that large orders neither overflow nor underflow).  Returns 0, or k>0 for a zero pivot.
  * Parity pinned by: free-field closed form, 2x2/3x3 explicit formulas. */
 int orc_zlogdet(int n, double *M /* col-major, overwritten */, double *logabs, double *arg);
 
@@ -83,7 +82,7 @@ int orc_cg_mdm(const orc_lattice *L, const double *A, const double *b, double *x
                double tol_abs_rr, double tol_rel, int max_iter, int *status);
 
 /* Noise (hmc.c:418-499).  phys_norm = 0 reproduces the reference
- * normalisation (Re,Im ~ N(0,1); defects D4/D5); 1 scales by 1/sqrt(2).
+ * normalisation (Re,Im ~ N(0,1)); 1 scales by 1/sqrt(2).
  * random_pseudofermion: returns sum |xi|^2, writes xi and psi = M^dag xi. */
 double orc_random_pseudofermion(const orc_lattice *L, orc_mt *st, const double *A,
                                 double *xi, double *psi, int phys_norm);
@@ -147,7 +146,7 @@ int orc_cg_propagator(const orc_bag_lattice *B, const int *field, double *x, con
                       double tol_abs_rr, int max_iter, int *status);
 
 /* Real partial-pivot LU: log|det| and sign (full_determinant.c:171-175
- * semantics, logs per D21).  M row-major or col-major (det is the same).
+ * semantics, accumulated as logs).  M row-major or col-major (det is the same).
  * Returns 0 or k>0 at an exactly zero pivot (then sign = 0). */
 int orc_dlogdet(int n, double *M, double *logabs, int *sign);
 

@@ -121,7 +121,7 @@ def test_fermion_matrix_and_determinant(hmc_lib, oracle):
 
 def test_random_fields_follow_the_reference_stream(hmc_lib, oracle):
     """random_pseudofermion, random_momentum, stochastic_vector draw the
-    reference's MT stream in its order and normalisation (D4/D5)."""
+    reference's MT stream in its order and normalisation (Re, Im ~ N(0,1))."""
     nt, nx = 8, 8
     L, A, _ = _setup(hmc_lib, oracle, nt, nx, 7)
     F = Links(nt, nx, A)

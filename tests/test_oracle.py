@@ -1,7 +1,7 @@
 """Pins of the CPU oracle (no GPU).
 
 Each check is independent of the oracle's own formulas: the reference's
-compiled RNG, the survey's recorded brute-force values, closed forms,
+compiled RNG, the brute-force values the reference's program printed, closed forms,
 identities of the operator and finite differences.
 """
 import json
@@ -257,8 +257,8 @@ def test_bag_weight_is_jacobi_ratio(oracle):
 
 def test_brute_table_matches_reference_run(oracle):
     """Exact enumeration on 4x4 OPENX, m=0, U=1, mu=0.1 (tests/parameter):
-    the values the reference's thirring_brute printed (recorded in the
-    survey of the reference): Z(0,0) = 1, Z(0,2) = 9.96905 (|Z| 10.0567),
+    the values the reference's tests/thirring_brute.c printed when it was
+    compiled and run on this lattice: Z(0,0) = 1, Z(0,2) = 9.96905 (|Z| 10.0567),
     Z(1,0) = 9.3976, Z(2,2) = 629.076, 45 non-zero cells."""
     B = oracle.bag_lattice(4, 4, bc=oracle.BC_OPENX, with_mass_monomers=1, m=0.0, mu=0.1)
     Z, Za, n = oracle.brute_enumerate(B)
