@@ -83,13 +83,15 @@ int th2d_gemm_batched(th2d_ctx *ctx, void *base, int precision, const th2d_gemm_
                       int count, int max_m, int max_n, double alpha);
 /* Dense kernels on MFMA (1, default) or on the vector ALUs (0). */
 int th2d_set_dense_mfma(int on);
-/* LU panel factorisation (process-wide): 1 (default) keeps a 16-column
- * panel in the registers of one 1024-thread block per matrix when every
- * order of the batch is at most 3072, with the row interchanges applied to
- * the other columns afterwards; 0 always uses the 32-column panel that
- * swaps whole rows as it goes.  Both give the factors of partial pivoting
- * with whole-row interchanges (LAPACK getrf); this build adds the switch
- * to compare them.  Replaces no reference function. */
+/* LU panel factorisation (process-wide): 1 (default), when every order of
+ * the batch is at most 3072, factors blocks of 128 columns as 16-column
+ * panels held in the registers of one block per matrix (256 to 1024
+ * threads), with the row interchanges applied to the other columns
+ * afterwards and one trailing update per block; 0 always uses the
+ * 32-column panel that swaps whole rows as it goes, with a trailing update
+ * per panel.  Both give the factors of partial pivoting with whole-row
+ * interchanges (LAPACK getrf); the switch is there to compare them.
+ * Replaces no reference function. */
 int th2d_set_lu_panel(int reg);
 int th2d_copy_f64_to_f32(th2d_ctx *ctx, float *dst, const double *src, long n);
 
