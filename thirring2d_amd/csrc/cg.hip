@@ -1527,7 +1527,7 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
     TH2D_HIP(hipLaunchCooperativeKernel(mu0 ? k_cg_resident<float, true> : k_cg_resident<float, false>,
                                         dim3(g.grid), dim3(RShape::NT), args, 0, ctx->stream));
   }
-  // results and the two flags in one copy
+  // results and the two flags into one pinned block, then one wait
   const size_t rb = sizeof(CgResult) * nch;
   TH2D_TRY(th2d_host_reserve(ctx->redh, rb + 64));
   TH2D_HIP(hipMemcpyAsync(ctx->redh.ptr, a.res, rb, hipMemcpyDeviceToHost, ctx->stream));
