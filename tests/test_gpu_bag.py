@@ -175,9 +175,10 @@ def test_lu_factors_match_lapack(bag, native, reg):
 
 
 def test_lu_logdet_f32_bound(bag, native):
-    """FP32-MFMA LU: log|det| within the stated bound of the FP64 value."""
+    """FP32-MFMA LU: log|det| within the stated bound of the FP64 value, on
+    orders up to 2304 (several 128-column blocks, every panel width)."""
     rng = np.random.default_rng(13)
-    ns = [16, 64, 100, 256, 400]
+    ns = [16, 64, 100, 256, 400, 1500, 2304]
     mats = _mats(rng, ns, shift=4.0)
     la64, sg64 = bag.DenseBatch(mats, native.PREC_F64).lu().logdet()
     la32, sg32 = bag.DenseBatch(mats, native.PREC_F32).lu().logdet()
