@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(256) k_lu_panel(T *base, BatchView bv, int *pi
 // row; after the barrier every thread picks the pivot from the 16 posts and
 // does the exchange, the scaling of the column and the rank-1 update of the
 // panel's later columns on registers.  Only the panel's columns are swapped
-// here; k_lu_swap applies the panel's interchanges to the other columns
+// here; k_lu_swap_trsm applies the panel's interchanges to the other columns
 // afterwards, in order, so the factors equal those of whole-row swaps.
 constexpr int PNB = 16;   // register panel width
 constexpr int PT = 1024;  // threads of the register panel
@@ -409,25 +409,52 @@ __global__ void __launch_bounds__(NT) k_lu_panel_reg(T *base, BatchView bv, int 
 }
 
 // The row interchanges of the panel at k (pivots k..k+jb-1, in order) on
-// every column outside it: one thread per column.
+// every column outside it, one thread per column; then, for the block's
+// later columns k + PNB <= c < c1, U of the panel's rows (as
+// k_lu_trsm<T, PNB>): the thread of a column applies both, so neither
+// waits for another thread.
 template <typename T>
-__global__ void __launch_bounds__(256) k_lu_swap(T *base, BatchView bv, const int *pivb, int k,
-                                                 int nb)
+__global__ void __launch_bounds__(256) k_lu_swap_trsm(T *base, BatchView bv, const int *pivb,
+                                                      int k, int c1)
 {
+  __shared__ T sL[PNB * PNB];
   const int b = blockIdx.y;
   const int n = bv.n[b];
-  if (k >= n) return;
-  const int jb = min(nb, n - k);
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= n || (c >= k && c < k + jb)) return;
+  if (k >= n) return;  // the whole workgroup
+  const int jb = min(PNB, n - k);
   T *A = base + bv.off[b];
   const int *piv = pivb + bv.poff[b];
-  for (int j = k; j < k + jb; j++) {
-    const int p = piv[j];
-    if (p != j) {
-      const T t0 = A[(long)j * n + c];
-      A[(long)j * n + c] = A[(long)p * n + c];
-      A[(long)p * n + c] = t0;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c < n && !(c >= k && c < k + jb)) {
+    for (int j = k; j < k + jb; j++) {
+      const int p = piv[j];
+      if (p != j) {
+        const T t0 = A[(long)j * n + c];
+        A[(long)j * n + c] = A[(long)p * n + c];
+        A[(long)p * n + c] = t0;
+      }
+    }
+  }
+  const int cb = k + PNB, ce = min(c1, n);
+  const int w0 = blockIdx.x * 256;
+  if (cb >= ce || w0 + 256 <= cb || w0 >= ce) return;  // the whole workgroup
+  for (int e = threadIdx.x; e < jb * jb; e += 256) {
+    const int r = e / jb, q = e - r * jb;
+    sL[r * PNB + q] = A[(long)(k + r) * n + k + q];
+  }
+  __syncthreads();
+  if (c < cb || c >= ce) return;
+  T x[PNB];
+#pragma unroll
+  for (int r = 0; r < PNB; r++) x[r] = r < jb ? A[(long)(k + r) * n + c] : (T)0;
+#pragma unroll
+  for (int r = 1; r < PNB; r++) {
+    if (r < jb) {
+      T sum = x[r];
+#pragma unroll
+      for (int q = 0; q < r; q++) sum -= sL[r * PNB + q] * x[q];
+      x[r] = sum;
+      A[(long)(k + r) * n + c] = sum;
     }
   }
 }
@@ -692,14 +719,10 @@ int lu_run(th2d_ctx *c, const th2d_batch *bt, T *A, int *piv, int *info)
         else
           hipLaunchKernelGGL((k_lu_panel_reg<T, PT>), dim3(cnt), dim3(PT), 0, c->stream, A, bv,
                              piv, info, kk);
-        hipLaunchKernelGGL(k_lu_swap<T>, dim3((N + 255) / 256, cnt), dim3(256), 0, c->stream, A,
-                           bv, piv, kk, PNB);
+        hipLaunchKernelGGL(k_lu_swap_trsm<T>, dim3((N + 255) / 256, cnt), dim3(256), 0,
+                           c->stream, A, bv, piv, kk, ce);
         const int cb = kk + PNB;
-        if (cb < ce) {
-          hipLaunchKernelGGL((k_lu_trsm<T, PNB>), dim3(1, cnt), dim3(256), 0, c->stream, A, bv,
-                             kk, cb, ce);
-          launch_update<T>(c, mf, A, bv, cnt, kk, PNB, cb, END, cb, ce, N);
-        }
+        if (cb < ce) launch_update<T>(c, mf, A, bv, cnt, kk, PNB, cb, END, cb, ce, N);
       }
       if (ce >= N) break;
       for (int s = k; s < ce; s += NB) {
