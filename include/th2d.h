@@ -279,6 +279,14 @@ int th2d_fermion_matrix(th2d_ctx *ctx, const th2d_geom *g, const double *K, doub
  * TH2D_ESINGULAR (log|det| = -inf) at an exactly zero pivot. */
 int th2d_zdet(th2d_ctx *ctx, int n, double *M, double *logabs, double *phase);
 
+/* The device generator's raw blocks: Philox4x32-10 (Salmon et al., SC'11)
+ * of n (counter, key) pairs, ctr_key = device uint32[6n] (4 counter words,
+ * then 2 key words, per item), out = device uint32[4n].  Exposed so the
+ * generator behind every device noise field can be checked against the
+ * published known-answer vectors; replaces no reference function (the
+ * reference draws MT19937 on the host, mersenne_inline.c:59-114). */
+int th2d_philox4x32_10(th2d_ctx *ctx, long n, const unsigned *ctr_key, unsigned *out);
+
 /* Gaussian N(0,1) link angles from the device generator (configs: A~N(0,1)). */
 int th2d_random_links(th2d_ctx *ctx, const th2d_geom *g, double *A, unsigned long long seed);
 

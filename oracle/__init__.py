@@ -56,6 +56,9 @@ def lib():
             "orc_mt_seed": (None, [P, C.c_long]),
             "orc_mt_next": (D, [P]),
             "orc_mt_skip": (None, [P, C.c_long]),
+            "orc_gauss_pair": (None, [P, P, P]),
+            "orc_philox4x32_10": (None, [P, P, P]),
+            "orc_philox_gauss2": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, P, P]),
             "orc_fm_mul": (None, [P, P, P, P]),
             "orc_fm_dag_mul": (None, [P, P, P, P]),
             "orc_fermion_matrix": (None, [P, P, P]),
@@ -78,6 +81,7 @@ def lib():
             "orc_dlogdet": (I, [I, P, P, P]),
             "orc_dinverse": (I, [I, P]),
             "orc_determinant_mu": (I, [P, P, D, P, P]),
+            "orc_reduced_matrix": (I, [P, P, D, P]),
             "orc_calc_Dinv": (I, [P, P]),
             "orc_bag_weight": (I, [P, P, P, P, P, P]),
             "orc_brute_enumerate": (C.c_long, [P, P, P]),
@@ -147,9 +151,44 @@ class Rng:
     def draws(self, n: int) -> np.ndarray:
         return np.array([self.next() for _ in range(n)])
 
+    def gauss_pairs(self, n: int) -> np.ndarray:
+        """n Box-Muller pairs (orc_gauss_pair) as an (n, 2) array."""
+        out = np.empty((n, 2))
+        c, s = D(), D()
+        f = lib().orc_gauss_pair
+        for k in range(n):
+            f(C.byref(self.st), C.byref(c), C.byref(s))
+            out[k, 0] = c.value
+            out[k, 1] = s.value
+        return out
+
     @property
     def ref(self):
         return C.byref(self.st)
+
+
+def philox4x32_10(ctr_key: np.ndarray) -> np.ndarray:
+    """Philox4x32-10 blocks of an (n, 6) uint32 array of (4 counter words,
+    2 key words); returns (n, 4) uint32."""
+    ck = np.ascontiguousarray(ctr_key, dtype=np.uint32).reshape(-1, 6)
+    out = np.empty((ck.shape[0], 4), np.uint32)
+    f = lib().orc_philox4x32_10
+    for j in range(ck.shape[0]):
+        row = ck[j]
+        f(_p(row[:4].copy()), _p(row[4:].copy()), _p(out[j]))
+    return out
+
+
+def philox_gauss(seed: int, stream: int, n: int, ctr: int = 0) -> np.ndarray:
+    """The device noise mapping: n complex N(0,1) draws for (seed, stream),
+    indices 0..n-1 at counter ctr."""
+    out = np.empty(n, np.complex128)
+    re, im = D(), D()
+    f = lib().orc_philox_gauss2
+    for i in range(n):
+        f(seed, stream, i, ctr, C.byref(re), C.byref(im))
+        out[i] = complex(re.value, im.value)
+    return out
 
 
 def lattice(nt, nx, m=0.1, g=0.5, mu=0.0, anti_t=1, anti_x=1, nf=2) -> Lattice:
@@ -320,6 +359,15 @@ def determinant_mu(B, field, mu=None):
     n = lib().orc_determinant_mu(C.byref(B), _p(field), B.mu if mu is None else mu,
                                  C.byref(la), C.byref(sg))
     return la.value, sg.value, n
+
+
+def reduced_matrix(B, field, mu=None):
+    """The fM_index matrix restricted to the free sites (natural order)."""
+    field = _field(field)
+    n = int(np.count_nonzero(field == 0))
+    M = np.empty(n * n, np.float64)
+    lib().orc_reduced_matrix(C.byref(B), _p(field), B.mu if mu is None else mu, _p(M))
+    return M.reshape(n, n)
 
 
 def calc_Dinv(B):

@@ -78,6 +78,46 @@ void orc_gauss_pair(orc_mt *st, double *c, double *s)
 }
 
 /* ------------------------------------------------------------------ */
+/* Philox4x32-10 and the device noise mapping                          */
+/* ------------------------------------------------------------------ */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int round = 0; round < 10; round++) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n1 = (uint32_t)p1;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    uint32_t n3 = (uint32_t)p0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;  /* Weyl key schedule */
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static double u53(uint32_t a, uint32_t b)
+{
+  uint64_t v = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
+  return ((double)v + 0.5) / 9007199254740992.0;
+}
+
+void orc_philox_gauss2(uint64_t seed, uint32_t stream, uint32_t idx, uint32_t ctr, double *re,
+                       double *im)
+{
+  const uint32_t c[4] = {idx, ctr, stream, 0x7468u};
+  const uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t r[4];
+  orc_philox4x32_10(c, k, r);
+  double u1 = u53(r[0], r[1]), u2 = u53(r[2], r[3]);
+  double rad = sqrt(-2.0 * log(u1));
+  *re = rad * cos(2 * M_PI * u2);
+  *im = rad * sin(2 * M_PI * u2);
+}
+
+/* ------------------------------------------------------------------ */
 /* U(1) staggered operator (hmc.c)                                     */
 /* ------------------------------------------------------------------ */
 int orc_eta(int t, int x, int dir)
@@ -688,21 +728,30 @@ int orc_dinverse(int n, double *M)
   return info;
 }
 
-int orc_determinant_mu(const orc_bag_lattice *B, const int *field, double mu,
-                       double *logabs, int *sign)
+int orc_reduced_matrix(const orc_bag_lattice *B, const int *field, double mu, double *M)
 {
-  int nt = B->nt, nx = B->nx, V = nt * nx;
+  int nx = B->nx, V = B->nt * nx;
   int *fr = malloc(sizeof(int) * V);
   int n = 0;
   for (int s = 0; s < V; s++)
     if (field[s] == 0) fr[n++] = s;
-  if (n == 0) { *logabs = 0.0; *sign = 1; free(fr); return 0; }
-  double *M = malloc(sizeof(double) * (size_t)n * n);
   for (int i = 0; i < n; i++)
     for (int j = 0; j < n; j++)
       M[(size_t)i * n + j] = orc_fM_index(B, fr[i] / nx, fr[i] % nx, fr[j] / nx, fr[j] % nx, mu);
+  free(fr);
+  return n;
+}
+
+int orc_determinant_mu(const orc_bag_lattice *B, const int *field, double mu,
+                       double *logabs, int *sign)
+{
+  int V = B->nt * B->nx, nfree = 0;
+  for (int s = 0; s < V; s++) nfree += field[s] == 0;
+  if (nfree == 0) { *logabs = 0.0; *sign = 1; return 0; }
+  double *M = malloc(sizeof(double) * (size_t)nfree * nfree);
+  int n = orc_reduced_matrix(B, field, mu, M);
   orc_dlogdet(n, M, logabs, sign);
-  free(M); free(fr);
+  free(M);
   return n;
 }
 

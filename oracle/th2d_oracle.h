@@ -49,8 +49,22 @@ void   orc_mt_seed(orc_mt *st, long seed);
 double orc_mt_next(orc_mt *st);
 void   orc_mt_skip(orc_mt *st, long n);
 
-/* Gaussian pair from two draws, x1 first (hmc.c:423-426). */
+/* Gaussian pair from two draws, x1 first (hmc.c:423-426).
+ * Parity pinned by: moments and a KS test of N(0,1) (test_gauss_pair_moments). */
 void orc_gauss_pair(orc_mt *st, double *c, double *s);
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): the counter-based
+ * generator behind the device noise fields (no reference counterpart; the
+ * reference draws MT on the host).  Parity pinned by: the published
+ * Random123 known-answer vectors (test_philox_known_answers). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* The device's Gaussian pair for (seed, stream, index, counter): one
+ * Philox block, two 53-bit uniforms in (0,1), Box-Muller.
+ * Parity pinned by: moments and a KS test of N(0,1)
+ * (test_philox_gauss_moments), the Philox known answers. */
+void orc_philox_gauss2(uint64_t seed, uint32_t stream, uint32_t idx, uint32_t ctr, double *re,
+                       double *im);
 
 /* Staggered phase eta_dir(t,x): eta_t = (-1)^x, eta_x = 1 (hmc.c:914-921) */
 int orc_eta(int t, int x, int dir);
@@ -152,6 +166,13 @@ int orc_dlogdet(int n, double *M, double *logabs, int *sign);
 
 /* In-place inverse of a real n x n matrix (row-major) via LU. */
 int orc_dinverse(int n, double *M);
+
+/* The dense reduced matrix over the unoccupied sites in natural order,
+ * M[i*n + j] = fM_index(free_i, free_j, mu) (full_determinant.c:157-167);
+ * M holds n_free^2 doubles.  Returns n_free.
+ * Parity pinned by: equal to the rows and columns of orc_fM applied to unit
+ * vectors (test_reduced_matrix_matches_fM). */
+int orc_reduced_matrix(const orc_bag_lattice *B, const int *field, double mu, double *M);
 
 /* Dense reduced matrix over unoccupied sites, its log|det| and sign
  * (full_determinant.c:144-186).  Returns n_free. */

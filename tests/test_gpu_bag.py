@@ -5,7 +5,7 @@ fluctuation_determinant.c, tests/thirring_brute.c).
 Tolerances: masked operator bit-exact-level (<= 1e-14 absolute); bag CG
 solution <= 1e-10 absolute at the reference stop r.r < 1e-26; FP64 LU
 log|det| <= 1e-11 relative and identical sign, FP64 inverse <= 1e-10
-relative (max |diff| / max |ref|); FP32 LU log|det| within 1e-4 * n
+relative (max |diff| / max |ref|); FP32 LU log|det| within 8 n eps32
 absolute of the FP64 value (stated mixed-precision bound) and identical
 sign; Dinv <= 1e-12; bag weights and Z table <= 1e-10 relative; rank-k
 ratios against full LU recomputes <= 1e-9 relative.
@@ -176,15 +176,22 @@ def test_lu_factors_match_lapack(bag, native, reg):
 
 def test_lu_logdet_f32_bound(bag, native):
     """FP32-MFMA LU: log|det| within the stated bound of the FP64 value, on
-    orders up to 2304 (several 128-column blocks, every panel width)."""
+    orders up to 2304 (several 128-column blocks, every panel width).
+
+    The FP32 factors are those of A + dA with |dA| <= gamma_n |L||U|, so
+    log|det| moves by tr(A^-1 dA), of order n eps32 for these shifted
+    Gaussian matrices (growth |U|/|A| <= 20).  LAPACK's sgetrf on the same
+    matrices is off by at most 1.1 n eps32; the bound is 8 n eps32
+    (2.2e-3 at n = 2304)."""
     rng = np.random.default_rng(13)
     ns = [16, 64, 100, 256, 400, 1500, 2304]
     mats = _mats(rng, ns, shift=4.0)
     la64, sg64 = bag.DenseBatch(mats, native.PREC_F64).lu().logdet()
     la32, sg32 = bag.DenseBatch(mats, native.PREC_F32).lu().logdet()
+    eps32 = float(np.finfo(np.float32).eps)
     for n, a, b, s1, s2 in zip(ns, la32, la64, sg32, sg64):
         assert s1 == s2
-        assert abs(a - b) <= 1e-4 * n
+        assert abs(a - b) <= 8 * n * eps32, (n, abs(a - b))
 
 
 def test_lu_singular(bag, native):

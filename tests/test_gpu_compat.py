@@ -115,6 +115,7 @@ def test_fermion_matrix_and_determinant(hmc_lib, oracle):
         assert abs(np.angle(z * np.exp(-1j * ph))) <= 1e-10
         s, l = np.linalg.slogdet(Mo)
         assert abs(l - la) <= 1e-10 * max(1.0, abs(l))
+        assert abs(np.angle(z / abs(z) * np.conj(s))) <= 1e-10
     finally:
         A_glob.value = None
 

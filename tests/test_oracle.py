@@ -301,3 +301,105 @@ def test_puregauge_sweep_samples_link_distribution(oracle):
     want = i1(beta) / i0(beta)
     err = np.std(acc) / np.sqrt(len(acc) / 5.0)  # sweeps are correlated
     assert abs(np.mean(acc) - want) <= 5 * err + 1e-3
+
+
+def test_gauss_pair_moments(oracle):
+    """orc_gauss_pair (Box-Muller over MT pairs, hmc.c:424-429) draws two
+    independent N(0,1): over 2e5 pairs the mean, variance, fourth moment and
+    the covariance of the two outputs lie within 5 sigma of N(0,1)'s, and a
+    Kolmogorov-Smirnov test against the normal CDF passes.  A slip such as
+    sqrt(-log x1) (variance 1/2) or a shared angle fails it."""
+    from scipy import stats
+    n = 200_000
+    z = oracle.Rng(2001, warmup=543210).gauss_pairs(n)
+    for k in range(2):
+        x = z[:, k]
+        assert abs(x.mean()) < 5 / np.sqrt(n)
+        assert abs((x * x).mean() - 1) < 5 * np.sqrt(2 / n)
+        assert abs((x ** 4).mean() - 3) < 5 * np.sqrt(96 / n)
+        assert stats.kstest(x, "norm").pvalue > 1e-4
+    assert abs((z[:, 0] * z[:, 1]).mean()) < 5 / np.sqrt(n)
+    # |z|^2 of a pair is exponential with mean 2 (the reference's S = |xi|^2)
+    r2 = (z * z).sum(axis=1)
+    assert abs(r2.mean() - 2) < 5 * 2 / np.sqrt(n)
+
+
+def test_zlogdet_phase_matches_numpy(oracle):
+    """orc_zlogdet (determinant, hmc.c:314-336, as log|det| and arg det)
+    against numpy's complex slogdet: the modulus and the phase, on random
+    complex matrices and on the fermion matrix at A != 0 and mu != 0."""
+    rng = np.random.default_rng(12)
+    mats = [rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) for n in (1, 2, 5, 17, 40)]
+    for nt, nx, mu in ((4, 4, 0.0), (6, 8, 0.3), (8, 8, -0.2)):
+        L = oracle.lattice(nt, nx, m=0.2, mu=mu)
+        mats.append(oracle.fermion_matrix(L, rng.normal(size=(nt * nx, 2))))
+    for M in mats:
+        la, ph, info = oracle.zlogdet(M)
+        s, l = np.linalg.slogdet(M)
+        assert info == 0
+        assert abs(la - l) <= 1e-11 * max(1.0, abs(l))
+        assert abs(np.angle(s * np.exp(-1j * ph))) <= 1e-10
+    # a permutation with one transposition has det -1: phase pi
+    P = np.eye(6, dtype=complex)[[1, 0, 2, 3, 4, 5]]
+    la, ph, info = oracle.zlogdet(P)
+    assert info == 0 and abs(la) < 1e-15 and abs(abs(ph) - np.pi) < 1e-15
+
+
+# Random123's published known-answer vectors for philox4x32 with 10 rounds
+# (counter words, key words, output words).
+PHILOX_KAT = [
+    ([0x00000000, 0x00000000, 0x00000000, 0x00000000], [0x00000000, 0x00000000],
+     [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff, 0xffffffff, 0xffffffff, 0xffffffff], [0xffffffff, 0xffffffff],
+     [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+]
+
+
+def test_philox_known_answers(oracle):
+    ck = np.array([c + k for c, k, _ in PHILOX_KAT], dtype=np.uint32)
+    out = oracle.philox4x32_10(ck)
+    np.testing.assert_array_equal(out, np.array([o for _, _, o in PHILOX_KAT], np.uint32))
+
+
+def test_philox_gauss_moments(oracle):
+    """The device noise mapping (one Philox block -> two 53-bit uniforms ->
+    Box-Muller) restated in the oracle: N(0,1) moments within 5 sigma and a
+    KS test, Re and Im uncorrelated, streams and counters independent."""
+    from scipy import stats
+    n = 100_000
+    z = oracle.philox_gauss(1002, 1, n)
+    w = oracle.philox_gauss(1002, 2, n)
+    for x in (z.real, z.imag, w.real):
+        assert abs(x.mean()) < 5 / np.sqrt(n)
+        assert abs((x * x).mean() - 1) < 5 * np.sqrt(2 / n)
+        assert abs((x ** 4).mean() - 3) < 5 * np.sqrt(96 / n)
+        assert stats.kstest(x, "norm").pvalue > 1e-4
+    assert abs((z.real * z.imag).mean()) < 5 / np.sqrt(n)
+    assert abs((z.real * w.real).mean()) < 5 / np.sqrt(n)
+    z1 = oracle.philox_gauss(1002, 1, 1000, ctr=1)
+    assert abs(np.mean(z[:1000].real * z1.real)) < 5 / np.sqrt(1000)
+
+
+def test_reduced_matrix_matches_fM(oracle):
+    """orc_reduced_matrix: column j is fM applied to the unit vector of free
+    site j, read at the free sites (vec_ops.c:96-131 drops hops into
+    occupied sites, so the free block of fM is the reduced matrix)."""
+    rng = np.random.default_rng(31)
+    for nt, nx, bc, wmm in ((4, 6, 0, 0), (6, 6, 1, 1), (8, 4, 0, 1)):
+        B = oracle.bag_lattice(nt, nx, bc=bc, with_mass_monomers=wmm, m=0.3, mu=0.2)
+        V = nt * nx
+        f = np.zeros(V, np.int32)
+        f[rng.choice(V, size=V // 4, replace=False)] = 1
+        free = np.flatnonzero(f == 0)
+        M = oracle.reduced_matrix(B, f)
+        for j, s in enumerate(free):
+            e = np.zeros(V)
+            e[s] = 1.0
+            assert np.array_equal(oracle.fM(B, f, e)[free], M[:, j])
+        la, sg, n = oracle.determinant_mu(B, f)
+        s_np, l_np = np.linalg.slogdet(M)
+        assert n == len(free) and sg == int(s_np)
+        if s_np != 0:
+            assert abs(la - l_np) < 1e-12 * max(1, abs(l_np))

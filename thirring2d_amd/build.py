@@ -10,6 +10,7 @@ import glob
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -42,8 +43,13 @@ def build(verbose: bool = False) -> str:
         cmd = ["gcc", "-O2", "-std=c99", "-fPIC", "-Wall", "-Wextra", "-c", src, "-o", obj]
         _run(cmd, verbose)
         objs.append(obj)
+    # One hipcc process per HIP source, run side by side, then one link.
+    hip_objs = [os.path.join(objdir, os.path.basename(s) + ".o") for s in sources()]
+    cmds = [[HIPCC] + HIP_FLAGS + ["-c", s, "-o", o] for s, o in zip(sources(), hip_objs)]
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        list(ex.map(lambda c: _run(c, verbose), cmds))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + sources() + objs
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + hip_objs + objs
     _run(cmd, verbose)
     os.replace(tmp, OUT)
     for lib, src in COMPAT_LIBS.items():

@@ -551,6 +551,30 @@ __device__ __forceinline__ double2 gauss2(unsigned long long seed, uint32_t stre
   return make_double2(rad * c, rad * s);
 }
 
+// Raw generator output for known-answer checks: item j has counter
+// ck[6j..6j+3] and key ck[6j+4..6j+5]; out[4j..4j+3] receives the block.
+__global__ void k_philox_blocks(long n, const uint32_t *ck, uint32_t *out)
+{
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t *q = ck + 6 * j;
+  uint4 r = philox10(make_uint4(q[0], q[1], q[2], q[3]), make_uint2(q[4], q[5]));
+  out[4 * j] = r.x;
+  out[4 * j + 1] = r.y;
+  out[4 * j + 2] = r.z;
+  out[4 * j + 3] = r.w;
+}
+
+extern "C" int th2d_philox4x32_10(th2d_ctx *c, long n, const unsigned *ctr_key, unsigned *out)
+{
+  if (!c || n < 0 || (n > 0 && (!ctr_key || !out))) return TH2D_EARG;
+  if (n == 0) return TH2D_OK;
+  hipLaunchKernelGGL(k_philox_blocks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                     n, (const uint32_t *)ctr_key, (uint32_t *)out);
+  TH2D_HIP(hipGetLastError());
+  return TH2D_OK;
+}
+
 // Gaussian fill of n complex numbers (or 2n reals) scaled by `scale`.
 __global__ void k_gauss_fill(double2 *v, long n, unsigned long long seed, uint32_t stream,
                              uint32_t ctr, double scale)
