@@ -201,6 +201,15 @@ int th2d_set_cg_graph(th2d_ctx *ctx, int on);
  * by the end of a pass; hence off by default. */
 int th2d_set_cg_reliable(th2d_ctx *ctx, int on);
 int th2d_cg_last_resident(th2d_ctx *ctx, int *on);
+/* Bounded waits of the resident solver.  Every wait between the regions
+ * of a chain gives up after `polls` polls (polls < 0: the default, 2^20,
+ * about 1 s) and ends every workgroup of the launch normally; the solve is
+ * then redone by the step kernel from the same right-hand sides, and
+ * th2d_cg_sync_fallbacks counts how often that happened on this context.
+ * A small limit forces the fallback from the host (used by the tests).
+ * Replaces no reference function (the reference's CG is serial). */
+int th2d_set_cg_spin_limit(th2d_ctx *ctx, long polls);
+int th2d_cg_sync_fallbacks(th2d_ctx *ctx, long *n);
 /* Solver launches enqueued between two reads of the stop flags
  * (0 = the iterations the same restart pass took in the previous solve,
  * then 8 at a time). */
@@ -294,6 +303,14 @@ int th2d_random_links(th2d_ctx *ctx, const th2d_geom *g, double *A, unsigned lon
  * per link in natural link order (update_puregauge_hb, hmc.c:82-93). */
 int th2d_puregauge_sweep(th2d_ctx *ctx, const th2d_geom *g, double *A, const double *draws_dev,
                          double gcoup, int nf);
+/* nsweeps of the same Metropolis update with draws from the device
+ * generator (sweep k of link l: one Philox block at index l, counter
+ * counter + k, stream 0x50475357; proposal first, accept draw second), all
+ * sweeps of a link in one launch: the thermalisation of hmc.c:927-929
+ * (100 sweeps from A = 0) without host draws.  Every link is updated
+ * independently, as the action Nf/g sum(1 - cos A) is a sum over links. */
+int th2d_puregauge_sweeps(th2d_ctx *ctx, const th2d_geom *g, double *A, double gcoup, int nf,
+                          int nsweeps, unsigned long long seed, unsigned long long counter);
 
 #ifdef __cplusplus
 }

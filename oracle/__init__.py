@@ -59,6 +59,7 @@ def lib():
             "orc_gauss_pair": (None, [P, P, P]),
             "orc_philox4x32_10": (None, [P, P, P]),
             "orc_philox_gauss2": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, P, P]),
+            "orc_philox_uniform2": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, P, P]),
             "orc_fm_mul": (None, [P, P, P, P]),
             "orc_fm_dag_mul": (None, [P, P, P, P]),
             "orc_fermion_matrix": (None, [P, P, P]),
@@ -188,6 +189,18 @@ def philox_gauss(seed: int, stream: int, n: int, ctr: int = 0) -> np.ndarray:
     for i in range(n):
         f(seed, stream, i, ctr, C.byref(re), C.byref(im))
         out[i] = complex(re.value, im.value)
+    return out
+
+
+def philox_uniform(seed: int, stream: int, n: int, ctr: int = 0) -> np.ndarray:
+    """(n, 2) uniform pairs of the device generator for (seed, stream),
+    indices 0..n-1 at counter ctr."""
+    out = np.empty((n, 2))
+    a, b = D(), D()
+    f = lib().orc_philox_uniform2
+    for i in range(n):
+        f(seed, stream, i, ctr, C.byref(a), C.byref(b))
+        out[i] = (a.value, b.value)
     return out
 
 

@@ -104,6 +104,17 @@ static double u53(uint32_t a, uint32_t b)
   return ((double)v + 0.5) / 9007199254740992.0;
 }
 
+void orc_philox_uniform2(uint64_t seed, uint32_t stream, uint32_t idx, uint32_t ctr, double *u0,
+                         double *u1)
+{
+  const uint32_t c[4] = {idx, ctr, stream, 0x7468u};
+  const uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t r[4];
+  orc_philox4x32_10(c, k, r);
+  *u0 = u53(r[0], r[1]);
+  *u1 = u53(r[2], r[3]);
+}
+
 void orc_philox_gauss2(uint64_t seed, uint32_t stream, uint32_t idx, uint32_t ctr, double *re,
                        double *im)
 {

@@ -66,6 +66,13 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 void orc_philox_gauss2(uint64_t seed, uint32_t stream, uint32_t idx, uint32_t ctr, double *re,
                        double *im);
 
+/* The device's uniform pair for (seed, stream, index, counter): the two
+ * 53-bit uniforms in (0,1) of one Philox block (the draws of the device
+ * pure-gauge sweeps).  Parity pinned by: the Philox known answers, and the
+ * uniform moments in test_philox_uniform_moments. */
+void orc_philox_uniform2(uint64_t seed, uint32_t stream, uint32_t idx, uint32_t ctr, double *u0,
+                         double *u1);
+
 /* Staggered phase eta_dir(t,x): eta_t = (-1)^x, eta_x = 1 (hmc.c:914-921) */
 int orc_eta(int t, int x, int dir);
 

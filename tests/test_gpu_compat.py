@@ -297,6 +297,47 @@ def test_bag_calc_dinv_signature(bag_lib, oracle, tmp_path, monkeypatch):
         C.c_void_p.in_dll(bag_lib, "field").value = None
 
 
+@pytest.mark.parametrize("bc", [0, 1])
+def test_bag_calc_dinv_without_mass_monomers(bag_lib, oracle, tmp_path, monkeypatch, bc):
+    """Without mass monomers calc_Dinv (fermion_matrix.c:355-425) gives the
+    full V x V inverse in natural order, Dinv[i1*V + i2] = (D^-1)_{i1,i2},
+    against the inverse of the oracle's fM_index matrix <= 1e-12; the cache
+    file is named for the lattice, m and mu (runs at different masses never
+    share it) and holds all V*V doubles; a short file is ignored."""
+    monkeypatch.chdir(tmp_path)
+    nt, nx = 6, 8
+    V = nt * nx
+    m, mu = 0.3, 0.1
+    field, B = _bag_setup(bag_lib, oracle, nt, nx, bc, 0, m, mu, np.zeros(V, np.int32))
+    buf = np.zeros(V * V)
+    g = C.c_void_p.in_dll(bag_lib, "Dinv")
+    g.value = buf.ctypes.data
+    try:
+        D_full = oracle.reduced_matrix(B, np.zeros(V, np.int32))
+        ref = np.linalg.inv(D_full).ravel()
+        name = f"free_propagator_T{nt}X{nx}_m{m:.6f}_mu{mu:.6f}" + ("_open" if bc else "")
+        cache = tmp_path / name
+        np.zeros(V * V // 2).tofile(cache)  # a half-length file: ignored
+        bag_lib.calc_Dinv()
+        assert np.max(np.abs(buf - ref)) <= 1e-12 * np.max(np.abs(ref))
+        assert cache.stat().st_size == 8 * V * V
+        assert np.array_equal(np.fromfile(cache, dtype=np.float64), buf)
+        changed = 2.0 * buf
+        changed.tofile(cache)
+        buf[:] = 0.0
+        bag_lib.calc_Dinv()
+        assert np.array_equal(buf, changed)
+        assert bag_lib.thirring_bag_last_status() == 0
+        # another mass is another file
+        C.c_double.in_dll(bag_lib, "m").value = 0.5
+        bag_lib.calc_Dinv()
+        assert (tmp_path / name.replace("m0.300000", "m0.500000")).exists()
+        assert not np.array_equal(buf, changed)
+    finally:
+        g.value = None
+        C.c_void_p.in_dll(bag_lib, "field").value = None
+
+
 def test_config_checkpoint(bag_lib, oracle, tmp_path, monkeypatch):
     """write_config / read_config (fermionbag.c:120-161): the file is named
     for the lattice, U, m and mu and holds NT*NX int32 at t*NX + x; reading
@@ -362,13 +403,14 @@ def test_bag_update_sequence(bag_lib, oracle):
         bag_lib.update_background()
         rng = np.random.default_rng(23)
         tried = acc = 0
+        kinds = set()
 
         def det(ff):
             la, sg, _ = oracle.determinant_mu(B, ff)
             return la, sg
 
         la_d, sg_d = det(np.zeros(V, np.int32))
-        for _ in range(200):
+        for _ in range(400):
             cur = field.natural().astype(np.int32)
             s = int(rng.integers(V))
             t, x = divmod(s, nx)
@@ -385,7 +427,17 @@ def test_bag_update_sequence(bag_lib, oracle):
                 kind = 2
                 r = bag_lib.det_removed_link(t, x, t2, x2)
                 new[s] = new[s2] = 0
-            elif cur[s] == 1 and cur[s2] == 0 and (t + x) % 2 == (t2 + x2) % 2:
+            elif cur[s] == 1 and cur[s2] == 1:
+                kind = 4
+                r = bag_lib.det_removed_link(t, x, t2, x2)
+                new[s] = new[s2] = 0
+            elif cur[s] in (1, 6) and rng.random() < 0.5:
+                # a monomer moves by two sites, to a site of its parity
+                dt, dx = [(2, 0), (0, 2), (1, 1), (1, -1)][int(rng.integers(4))]
+                t2, x2 = (t + dt) % nt, (x + dx) % nx
+                s2 = t2 * nx + x2
+                if cur[s2] != 0:
+                    continue
                 kind = 3
                 r = bag_lib.det_moved_monomer(t, x, t2, x2)
                 new[s], new[s2] = 0, 6
@@ -404,13 +456,17 @@ def test_bag_update_sequence(bag_lib, oracle):
                 elif kind == 2:
                     bag_lib.removed_link(t, x, nu)
                     want_links -= 1
+                elif kind == 4:
+                    bag_lib.removed_monomer(t, x, nu)
+                    want_mono -= 2
                 else:
                     bag_lib.moved_source_monomer(t, x, t2, x2)
+                kinds.add(kind)
                 acc += 1
                 assert np.array_equal(field.natural(), new)
                 assert sign.value == sg1 * sg_d
         assert bag_lib.thirring_bag_last_status() == 0
-        assert tried > 40 and acc > 20
+        assert tried > 40 and acc > 20 and kinds == {0, 1, 2, 3, 4}
         assert nmono.value == want_mono and nlinks.value == want_links
     finally:
         C.c_void_p.in_dll(bag_lib, "field").value = None

@@ -384,6 +384,41 @@ def test_cg_resident_full_device(hmc, oracle):
         assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
 
 
+def test_cg_resident_wait_timeout_falls_back(hmc, oracle):
+    """A bounded wait of the resident solver that gives up (forced from the
+    host with a spin limit of 0 polls: every workgroup still exits normally)
+    makes the solve fall back to the step kernel from the same right-hand
+    sides: the solve and a batched trajectory both finish with every chain
+    at its true residual, and the fallback is counted.  With the default
+    limit the resident kernel runs again."""
+    import ctypes as C
+    from thirring2d_amd import _native as N
+    setups = [_setup(hmc, oracle, 128, 128, 900 + k, m=0.1) for k in range(4)]
+    batch = hmc.ChainBatch([s[0] for s in setups])
+    ctx = setups[0][0].ctx
+    p = hmc.cg_params(tol_rel=1e-10, precision=N.PREC_MIXED)
+    n0, n1 = C.c_long(), C.c_long()
+    N.check(N.lib().th2d_cg_sync_fallbacks(ctx.ptr, C.byref(n0)), "fallbacks")
+    N.check(N.lib().th2d_set_cg_spin_limit(ctx.ptr, 0), "spin")
+    try:
+        xs, infos = batch.fmdm_invert_cg([s[3] for s in setups], p)
+        assert _last_resident(ctx) == 0
+        for (lat, L, A, b, _), x, inf in zip(setups, xs, infos):
+            assert inf.status == 0 and inf.true_rel_resid <= 1e-10
+            r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
+            assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
+        res = batch.update_gauge(nsteps=2, traj_len=0.1, seed=5, counters=[0, 1, 2, 3], params=p)
+        for r in res:
+            assert r.status == 0 and r.max_true_rel_resid <= 1e-10
+        N.check(N.lib().th2d_cg_sync_fallbacks(ctx.ptr, C.byref(n1)), "fallbacks")
+        assert n1.value - n0.value >= 4  # the solve and the trajectory's 3 solves
+    finally:
+        N.check(N.lib().th2d_set_cg_spin_limit(ctx.ptr, -1), "spin")
+    xs, infos = batch.fmdm_invert_cg([s[3] for s in setups], p)
+    assert _last_resident(ctx) == 1
+    assert all(inf.status == 0 and inf.true_rel_resid <= 1e-10 for inf in infos)
+
+
 def test_cg_resident_groups(hmc, oracle):
     """A batch with more regions than the device holds resident workgroups
     (12 x 256^2 + 20 x 128^2 = 272 regions) is solved by the resident kernel

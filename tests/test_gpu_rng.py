@@ -109,3 +109,51 @@ def test_trajectory_noise_moments(hmc, phys_norm):
     assert abs(acts[:, 3].mean() - 2 * V * s2) < 5 * sd
     assert abs(acts[:, 2].mean() - 2 * m * V * s2) < 5 * m * sd
     assert len(set(acts[:, 1])) == 4 and len(set(acts[:, 3])) == 4
+
+
+def _eo_link(nt, nx, s, d):
+    t, x = divmod(s, nx)
+    V = nt * nx
+    return d * V + ((t + x) & 1) * (V // 2) + (t * nx + x) // 2
+
+
+def test_puregauge_sweeps_match_host_draw_sweep(hmc, oracle):
+    """th2d_puregauge_sweeps (device draws) against th2d_puregauge_sweep
+    (update_puregauge_hb, hmc.c:82-93, draws supplied) given the same draws,
+    restated by the oracle for every link: identical fields after one
+    sweep."""
+    from thirring2d_amd import _native as N
+    nt, nx, seed, ctr = 8, 12, 91, 5
+    V = nt * nx
+    A0 = np.random.default_rng(8).normal(size=(V, 2))
+    a = hmc.StaggeredLattice(nt, nx, g=0.5)
+    a.set_links(A0)
+    N.check(N.lib().th2d_puregauge_sweeps(a.ctx.ptr, a._g(), a.A.p, 0.5, 2, 1, seed, ctr), "sw")
+    U = oracle.philox_uniform(seed, 0x50475357, 2 * V, ctr=ctr)
+    draws = np.empty(4 * V)
+    for s in range(V):
+        for d in range(2):
+            l = _eo_link(nt, nx, s, d)
+            draws[4 * s + 2 * d], draws[4 * s + 2 * d + 1] = U[l]
+    b = hmc.StaggeredLattice(nt, nx, g=0.5)
+    b.set_links(A0)
+    dd = hmc.DeviceArray.from_host(draws)
+    N.check(N.lib().th2d_puregauge_sweep(b.ctx.ptr, b._g(), b.A.p, dd.p, 0.5, 2), "sweep")
+    Aa, Ab = a.get_links(), b.get_links()
+    assert np.array_equal(Aa, Ab)
+    assert 0.1 < np.mean(Aa != A0) < 0.9  # some proposals accepted, some not
+
+
+def test_puregauge_thermalization_distribution(hmc):
+    """100 device sweeps from A = 0 (hmc.c:927-929): every link follows
+    exp(beta cos A), beta = Nf/g, so <cos A> = I1(beta)/I0(beta) within
+    5 sigma over the 32,768 links of 128^2 (g = 0.5 and g = 1.0)."""
+    from scipy.special import i0, i1
+    for g in (0.5, 1.0):
+        lat = hmc.StaggeredLattice(128, 128, g=g)
+        lat.thermalize_pure_gauge(sweeps=100, seed=11)
+        c = np.cos(lat.get_links()).ravel()
+        beta = 2 / g
+        want = i1(beta) / i0(beta)
+        var = 0.5 * (1 + (i0(beta) - 2 * i1(beta) / beta) / i0(beta)) - want ** 2
+        assert abs(c.mean() - want) < 5 * np.sqrt(var / c.size), (g, c.mean(), want)

@@ -403,3 +403,17 @@ def test_reduced_matrix_matches_fM(oracle):
         assert n == len(free) and sg == int(s_np)
         if s_np != 0:
             assert abs(la - l_np) < 1e-12 * max(1, abs(l_np))
+
+
+def test_philox_uniform_moments(oracle):
+    """The device uniform pairs: mean 1/2, variance 1/12, strictly inside
+    (0,1), the two members uncorrelated, a KS test against U(0,1)."""
+    from scipy import stats
+    n = 100_000
+    u = oracle.philox_uniform(77, 0x50475357, n, ctr=3)
+    assert u.min() > 0.0 and u.max() < 1.0
+    for k in range(2):
+        assert abs(u[:, k].mean() - 0.5) < 5 * np.sqrt(1 / 12 / n)
+        assert abs(u[:, k].var() - 1 / 12) < 5 * np.sqrt(1 / 180 / n)
+        assert stats.kstest(u[:, k], "uniform").pvalue > 1e-4
+    assert abs(np.corrcoef(u[:, 0], u[:, 1])[0, 1]) < 5 / np.sqrt(n)

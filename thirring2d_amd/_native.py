@@ -112,6 +112,7 @@ SIGNATURES = {
     "th2d_philox4x32_10": (I, [P, L, P, P]),
     "th2d_random_links": (I, [P, C.POINTER(Geom), P, U64]),
     "th2d_puregauge_sweep": (I, [P, C.POINTER(Geom), P, P, D, I]),
+    "th2d_puregauge_sweeps": (I, [P, C.POINTER(Geom), P, D, I, I, U64, U64]),
     "th2d_set_cg_tile": (I, [P, I, I]),
     "th2d_cg_last_tile": (I, [P, C.POINTER(I), C.POINTER(I)]),
     "th2d_set_cg_autotune": (I, [P, I]),
@@ -119,6 +120,8 @@ SIGNATURES = {
     "th2d_set_cg_graph": (I, [P, I]),
     "th2d_set_cg_reliable": (I, [P, I]),
     "th2d_cg_last_resident": (I, [P, C.POINTER(I)]),
+    "th2d_set_cg_spin_limit": (I, [P, L]),
+    "th2d_cg_sync_fallbacks": (I, [P, C.POINTER(L)]),
     "th2d_set_dslash_variant": (I, [I]),
     "th2d_set_cg_max_blocks": (I, [P, I]),
     "th2d_set_cg_chunk": (I, [P, I]),

@@ -141,41 +141,65 @@ double fM_index(int t1, int x1, int t2, int x2, double mu_)
   return v;
 }
 
-/* The free propagator is cached on disk as fermion_matrix.c:206-220 and
- * 342-346 do: a file named for the lattice, mu and the boundary condition
- * in the working directory holds the V*V/2 doubles of Dinv (its eo block
- * layout, host byte order).  A file that exists is read instead of
+/* The free propagator is cached on disk as fermion_matrix.c:206-220,
+ * 342-346 and 376-384, 418-422 do, in the working directory, host byte
+ * order.  A file that exists with the expected length is read instead of
  * computing; a short file is ignored here (the reference reads it
- * unchecked).  Otherwise Dinv is computed on the GPU and the file written. */
+ * unchecked).  Otherwise Dinv is computed on the GPU and the file written.
+ *
+ * With mass monomers (fermion_matrix.c:191-351) Dinv holds the eo blocks,
+ * V*V/2 doubles, in a file named for the lattice, mu and the boundary.
+ * Without them (:355-425) Dinv is the full V x V inverse in natural site
+ * order, Dinv[(NX*t1 + x1)*V + NX*t2 + x2] = (D^-1)_{1,2}, and the file
+ * name also holds m.  The reference writes only the first V*V/2 doubles of
+ * that inverse (and reads V*V back); this library writes all V*V, so its
+ * files read back whole, and it ignores the reference's half-length ones. */
 void calc_Dinv(void)
 {
   if (!ready()) return;
-  const size_t half = (size_t)S.V * S.V / 2;
+  const int eo = S.geo.with_mass_monomers != 0;
+  const size_t count = eo ? (size_t)S.V * S.V / 2 : (size_t)S.V * S.V;
   if (!Dinv) Dinv = malloc(sizeof(double) * (size_t)S.V * S.V);
   if (!Dinv) {
     S.status = TH2D_ENOMEM;
     return;
   }
-  char filename[100];
-  snprintf(filename, sizeof(filename), S.geo.bc == 1 ? "free_propagator_eo_T%dX%d_mu%0.6f_open"
-                                                      : "free_propagator_eo_T%dX%d_mu%0.6f",
-           S.nt, S.nx, mu);
+  char filename[128];
+  const char *open = S.geo.bc == 1 ? "_open" : "";
+  if (eo)
+    snprintf(filename, sizeof(filename), "free_propagator_eo_T%dX%d_mu%0.6f%s", S.nt, S.nx, mu,
+             open);
+  else
+    snprintf(filename, sizeof(filename), "free_propagator_T%dX%d_m%.6f_mu%0.6f%s", S.nt, S.nx, m,
+             mu, open);
   FILE *f = fopen(filename, "rb");
   if (f) {
-    const size_t got = fread(Dinv, sizeof(double), half, f);
+    const size_t got = fread(Dinv, sizeof(double), count, f);
     fclose(f);
-    if (got == half) return;
+    if (got == count) return;
   }
   struct timeval start, end;
   gettimeofday(&start, NULL);
-  if (fail(th2d_bag_calc_dinv(S.ctx, &S.geo, S.dD))) return;
-  if (fail(th2d_d2h(S.ctx, Dinv, S.dD, sizeof(double) * half))) return;
+  if (eo) {
+    if (fail(th2d_bag_calc_dinv(S.ctx, &S.geo, S.dD))) return;
+    if (fail(th2d_d2h(S.ctx, Dinv, S.dD, sizeof(double) * count))) return;
+  } else {
+    /* B = D^-1 of the empty configuration, V x V row-major, natural order */
+    th2d_bag *b = NULL;
+    double *B = NULL;
+    int *fd = NULL, V = 0;
+    if (fail(th2d_bag_create(S.ctx, &S.geo, &b))) return;
+    int rc = th2d_bag_views(b, &B, &fd, &V);
+    if (!rc) rc = th2d_d2h(S.ctx, Dinv, B, sizeof(double) * count);
+    th2d_bag_destroy(b);
+    if (fail(rc)) return;
+  }
   gettimeofday(&end, NULL);
   const long diff = 1000000L * (end.tv_sec - start.tv_sec) + (end.tv_usec - start.tv_usec);
   printf("Inverted fermion matrix in %.3g seconds\n", 1e-6 * diff);
   f = fopen(filename, "wb");
   if (f) {
-    fwrite(Dinv, sizeof(double), half, f);
+    fwrite(Dinv, sizeof(double), count, f);
     fclose(f);
   }
 }

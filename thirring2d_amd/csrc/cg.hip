@@ -476,6 +476,7 @@ struct ResCtl {
   unsigned blk_bytes;  // bytes of the solver block from CgBatch::x (buffer range)
   unsigned zero;     // 0; an addend the compiler cannot see, so that a poll
                      // stays a read-modify-write atomic (executed in L2)
+  unsigned spin;     // polls of one wait before it gives up (RES_SPIN_LIMIT)
 };
 
 __device__ __forceinline__ void pub_store(float2 *p, float2 v)
@@ -547,7 +548,8 @@ struct ResMem {
   // Arrive on the counter at byte offset `obar` and wait until it reaches
   // `target`; false when the wait timed out here or elsewhere (then the
   // error flag is set).  Every wave first waits for its own stores.
-  __device__ bool sync(unsigned obar, unsigned target, int *err, double *flag, unsigned zero) const
+  __device__ bool sync(unsigned obar, unsigned target, int *err, double *flag, unsigned zero,
+                       unsigned spin) const
   {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores have completed
     __syncthreads();
@@ -558,7 +560,7 @@ struct ResMem {
       for (unsigned n = 0;; n++) {
         if (__hip_atomic_fetch_add(bar, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= target)
           break;
-        if (n >= RES_SPIN_LIMIT ||
+        if (n >= spin ||
             ((n & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
           __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           ok = false;
@@ -578,7 +580,7 @@ struct ResMem {
 // are known to share an XCD: the caller's stores have completed before it
 // arrives, and after the wait an acquire fence (L1 and L2 invalidation)
 // orders the reads that follow.
-__device__ bool agent_sync(unsigned *bar, unsigned target, int *err, double *flag)
+__device__ bool agent_sync(unsigned *bar, unsigned target, int *err, double *flag, unsigned spin)
 {
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores have completed
   __syncthreads();
@@ -587,7 +589,7 @@ __device__ bool agent_sync(unsigned *bar, unsigned target, int *err, double *fla
     bool ok = true;
     for (unsigned n = 0; __hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target;
          n++) {
-      if (n >= RES_SPIN_LIMIT ||
+      if (n >= spin ||
           ((n & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
         __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ok = false;
@@ -796,7 +798,7 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   // the batch with the step kernel)
   if (tid == 0)
     __hip_atomic_store(ctl.xcc + tau, xcc_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (!agent_sync(ctl.bar0 + 32 * tl.chain, (unsigned)ntc, ctl.err, red + 36)) return;
+  if (!agent_sync(ctl.bar0 + 32 * tl.chain, (unsigned)ntc, ctl.err, red + 36, ctl.spin)) return;
   if (tid == 0) {
     const int mine = __hip_atomic_load(ctl.xcc + tau, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     bool same = true;
@@ -947,7 +949,8 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
     slot = ns;
     setup = false;
     // 1. every region of the chain has finished iteration j
-    if (!M.sync(o_bar, (unsigned)(j + 2) * (unsigned)ntc, ctl.err, red + 36, ctl.zero)) break;
+    if (!M.sync(o_bar, (unsigned)(j + 2) * (unsigned)ntc, ctl.err, red + 36, ctl.zero, ctl.spin))
+      break;
     // 2. the chain's partials of iteration j+1, summed by every wave in the
     // same order (a butterfly leaves the total in every lane), so that all
     // waves of all regions of the chain hold bitwise the same gam, del; then
@@ -1514,6 +1517,7 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   if (blk >= (size_t)1 << 31) return TH2D_ELAUNCH;  // buffer offsets are 32-bit
   ctl.blk_bytes = (unsigned)blk;
   ctl.zero = 0;
+  ctl.spin = ctx->res_spin_limit < 0 ? RES_SPIN_LIMIT : (unsigned)ctx->res_spin_limit;
   if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k0, ctx->stream));
   hipLaunchKernelGGL(k_zero_u32, dim3(std::min<int>(64, (int)(nzero + 255) / 256)), dim3(256), 0,
                      ctx->stream, ctl.bar, (int)nzero);
@@ -1624,7 +1628,12 @@ int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
         rc = resident_solve(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, xout, res, carry);
       else if (!ra.empty())
         rc = split_solve(ctx, buf, pass, G, rhs, K, stop_rr, max_iter, xout, res, carry, ra, rb);
-      if (rc != TH2D_ELAUNCH) return rc;
+      // A wait that timed out (TH2D_ESYNC) left every workgroup exiting
+      // normally and the right-hand sides untouched: the step kernel
+      // re-solves the batch, as it does for a batch the resident kernel
+      // declines (TH2D_ELAUNCH).
+      if (rc == TH2D_ESYNC) ctx->cg_sync_fallbacks++;
+      if (rc != TH2D_ELAUNCH && rc != TH2D_ESYNC) return rc;
     }
   }
   ctx->cg_last_resident = 0;
@@ -1752,6 +1761,20 @@ extern "C" int th2d_cg_last_resident(th2d_ctx *ctx, int *on)
 {
   if (!ctx || !on) return TH2D_EARG;
   *on = ctx->cg_last_resident;
+  return TH2D_OK;
+}
+
+extern "C" int th2d_set_cg_spin_limit(th2d_ctx *ctx, long polls)
+{
+  if (!ctx || polls > (long)0xFFFFFFFFu) return TH2D_EARG;
+  ctx->res_spin_limit = polls < 0 ? -1 : polls;
+  return TH2D_OK;
+}
+
+extern "C" int th2d_cg_sync_fallbacks(th2d_ctx *ctx, long *n)
+{
+  if (!ctx || !n) return TH2D_EARG;
+  *n = ctx->cg_sync_fallbacks;
   return TH2D_OK;
 }
 

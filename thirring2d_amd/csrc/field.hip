@@ -624,6 +624,49 @@ __global__ void k_puregauge(Geo G, double *A, const double *draws, double cg)
   }
 }
 
+// n pure-gauge Metropolis sweeps with device draws.  The action
+// Nf/g sum(1 - cos A) is a sum of single-link terms, so every link runs its
+// n sweeps in registers; sweep k of link l draws one Philox block at
+// (index l, counter ctr0 + k): the first 53-bit uniform is the proposal,
+// the second the accept draw (hmc.c:82-93 order).
+__device__ __forceinline__ void uniform2(unsigned long long seed, uint32_t stream, uint32_t idx,
+                                         uint32_t ctr, double *u0, double *u1)
+{
+  uint4 r = philox10(make_uint4(idx, ctr, stream, 0x7468u),
+                     make_uint2((uint32_t)seed, (uint32_t)(seed >> 32)));
+  *u0 = u01(r.x, r.y);
+  *u1 = u01(r.z, r.w);
+}
+
+__global__ void k_puregauge_n(long nlinks, double *A, double cg, int nsweeps,
+                              unsigned long long seed, uint32_t ctr0)
+{
+  long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nlinks) return;
+  double a = A[l];
+  for (int k = 0; k < nsweeps; k++) {
+    double u0, u1;
+    uniform2(seed, 0x50475357u, (uint32_t)l, ctr0 + (uint32_t)k, &u0, &u1);
+    const double nw = 2 * M_PI * u0 - M_PI;
+    if (u1 < exp(-cg * cos(a) + cg * cos(nw))) a = nw;
+  }
+  A[l] = a;
+}
+
+extern "C" int th2d_puregauge_sweeps(th2d_ctx *c, const th2d_geom *g, double *A, double gc,
+                                     int nf, int nsweeps, unsigned long long seed,
+                                     unsigned long long counter)
+{
+  TH2D_TRY(th2d_check_geom(g));
+  if (!c || !A || gc == 0.0 || nsweeps < 0) return TH2D_EARG;
+  if (nsweeps == 0) return TH2D_OK;
+  const long nl = 2L * g->nt * g->nx;
+  hipLaunchKernelGGL(k_puregauge_n, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, c->stream,
+                     nl, A, nf / gc, nsweeps, seed, (uint32_t)counter);
+  TH2D_HIP(hipGetLastError());
+  return TH2D_OK;
+}
+
 extern "C" int th2d_puregauge_sweep(th2d_ctx *c, const th2d_geom *g, double *A,
                                     const double *draws, double gc, int nf)
 {

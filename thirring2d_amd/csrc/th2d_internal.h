@@ -65,6 +65,8 @@ struct th2d_ctx {
   int cg_reliable = 0;            // mixed solves continue the Krylov sequence across passes
   int res_capacity = -1;          // resident workgroups the device holds (-1: not queried)
   int cg_last_resident = 0;       // the last inner solve: 0 step kernel, 1 resident, 2 both
+  long res_spin_limit = -1;       // polls of one resident-kernel wait (-1: the default)
+  long cg_sync_fallbacks = 0;     // resident solves that timed out and were re-solved
   std::map<unsigned long long, int> cg_tuned[2];  // signature -> shape (FP32, FP64)
   // step-kernel launches captured as graphs (cg.hip), keyed by the launch
   // arguments; cg_graph 0 (default) launches eagerly

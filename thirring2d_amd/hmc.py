@@ -135,6 +135,15 @@ class StaggeredLattice:
         N.check(N.lib().th2d_random_links(self.ctx.ptr, self._g(), self.A.p, seed), "rand")
         self.refold()
 
+    def thermalize_pure_gauge(self, sweeps=100, seed=0, counter=0, from_zero=True):
+        """A = 0, then `sweeps` pure-gauge Metropolis sweeps with device
+        draws (hmc.c:927-929 with update_puregauge_hb, hmc.c:82-93)."""
+        if from_zero:
+            self.set_links(np.zeros((self.V, 2)))
+        N.check(N.lib().th2d_puregauge_sweeps(self.ctx.ptr, self._g(), self.A.p, self.g,
+                                              self.nf, sweeps, seed, counter), "puregauge")
+        self.refold()
+
     def to_eo(self, v: np.ndarray, dst: DeviceArray):
         self._stage.upload(np.ascontiguousarray(v, dtype=np.complex128))
         N.check(N.lib().th2d_vec_to_eo(self.ctx.ptr, self._g(), dst.p, self._stage.p), "to_eo")
