@@ -60,6 +60,20 @@ int thirring_hmc_last_status(void);
 /* Acceptance of the last update_gauge, its dH and CG iterations. */
 int thirring_hmc_last_trajectory(int *accepted, double *dH, int *cg_iterations);
 
+/* ---- checkpoint (no counterpart in the reference, whose HMC runs cannot
+ * resume: hmc.c:932-939) ---------------------------------------------------
+ * Write the field A (FP64), m, g, mu, the caller's trajectory counter and
+ * the whole MT19937 state to `path`; reading it back (same lattice extents)
+ * restores A, m, g, mu, the counter and the random stream, so the next
+ * update_gauge repeats the trajectory the uninterrupted run would have
+ * made, bit for bit.  Returns 0, TH2D_EARG (I/O error, other lattice, not a
+ * checkpoint) or, on reading, THIRRING_HMC_RNG_PARTIAL when the program
+ * supplies its own generator (mersenne_inline.c): its state words are
+ * file-static, so only the current block of 624 draws is restored. */
+#define THIRRING_HMC_RNG_PARTIAL 1
+int thirring_hmc_write_checkpoint(const char *path, double ***A, long trajectory);
+int thirring_hmc_read_checkpoint(const char *path, double ***A, long *trajectory);
+
 /* ---- hmc.c ------------------------------------------------------------ */
 double calc_gauge_action(double ***A);                                  /* :73-79   */
 double gauge_step(double ***A, double ***mom, double eps);              /* :97-101  */

@@ -46,6 +46,7 @@ def hmc_lib():
     lib.thirring_hmc_set_md.argtypes = [I, D]
     lib.thirring_hmc_last_trajectory.argtypes = [C.POINTER(I), C.POINTER(D), C.POINTER(I)]
     lib.seed_mersenne.argtypes = [C.c_long]
+    lib.mersenne_generate.restype = D
     return lib
 
 
@@ -196,6 +197,69 @@ def test_update_gauge_matches_oracle(hmc_lib, oracle):
             n_acc += acco
         assert hmc_lib.thirring_hmc_last_status() == 0
         assert n_acc >= 1
+    finally:
+        hmc_lib.thirring_hmc_set_cg(0.0, 1e-30, 100000, 0)
+        hmc_lib.thirring_hmc_set_md(10, 1.0)
+
+
+def test_hmc_checkpoint_resumes_bitwise(hmc_lib, oracle, tmp_path):
+    """thirring_hmc_write_checkpoint after two trajectories, then a third;
+    a fresh start (other field, other seed) that reads the checkpoint
+    repeats the third trajectory bit for bit: dH, accept and field.  The MT
+    state in the file equals the stream's (the next draws of a generator
+    restored from it match the oracle's stream)."""
+    nt, nx = 16, 16
+    L, A, _ = _setup(hmc_lib, oracle, nt, nx, 31, m=0.1, g=0.5)
+    hmc_lib.thirring_hmc_set_cg(1e-12, 0.0, 100000, 0)
+    hmc_lib.thirring_hmc_set_md(40, 1.0)
+    hmc_lib.thirring_hmc_write_checkpoint.argtypes = [C.c_char_p, P, C.c_long]
+    hmc_lib.thirring_hmc_read_checkpoint.argtypes = [C.c_char_p, P, C.POINTER(C.c_long)]
+    path = str(tmp_path / "hmc.ckpt").encode()
+
+    def traj(F):
+        hmc_lib.update_gauge(F.ptr)
+        acc, dH, its = I(), D(), I()
+        hmc_lib.thirring_hmc_last_trajectory(C.byref(acc), C.byref(dH), C.byref(its))
+        assert hmc_lib.thirring_hmc_last_status() == 0
+        return acc.value, dH.value
+
+    try:
+        F = Links(nt, nx, A)
+        hmc_lib.seed_mersenne(2001)
+        for _ in range(2):
+            traj(F)
+        assert hmc_lib.thirring_hmc_write_checkpoint(path, F.ptr, 2) == 0
+        acc3, dH3 = traj(F)
+        A3 = F.natural()
+        # the file's stream: 3 x 2V Gaussian pairs + 1 accept draw per
+        # trajectory consumed from seed 2001 so far, twice
+        rng = oracle.Rng(2001)
+        rng.draws(2 * (3 * 2 * nt * nx + 1))
+        # start over from somewhere else, then resume
+        _globals(hmc_lib, 0.7, 0.9, 0.0)
+        G = Links(nt, nx, np.zeros((nt * nx, 2)))
+        hmc_lib.seed_mersenne(77)
+        traj(G)
+        n = C.c_long(-1)
+        assert hmc_lib.thirring_hmc_read_checkpoint(path, G.ptr, C.byref(n)) == 0
+        assert n.value == 2
+        assert C.c_double.in_dll(hmc_lib, "m").value == 0.1
+        acc, dH = traj(G)
+        assert acc == acc3 and dH == dH3
+        assert np.array_equal(G.natural(), A3)
+        # and the generator continues the reference stream draw for draw
+        rng.draws(3 * 2 * nt * nx + 1)
+        mi = C.c_int.in_dll(hmc_lib, "mersenne_i")
+        arr = (C.c_double * 624).in_dll(hmc_lib, "mersenne_array")
+        nxt = []
+        for _ in range(700):
+            if mi.value > 0:
+                mi.value -= 1
+                nxt.append(arr[mi.value])
+            else:
+                nxt.append(hmc_lib.mersenne_generate())
+        assert np.array_equal(np.array(nxt), rng.draws(700))
+        assert hmc_lib.thirring_hmc_read_checkpoint(b"/nonexistent/x", G.ptr, C.byref(n)) != 0
     finally:
         hmc_lib.thirring_hmc_set_cg(0.0, 1e-30, 100000, 0)
         hmc_lib.thirring_hmc_set_md(10, 1.0)

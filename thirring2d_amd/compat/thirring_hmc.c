@@ -8,6 +8,7 @@
 #include "../../include/thirring_hmc.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -29,7 +30,7 @@ __attribute__((weak)) int mersenne_i = -1;
 __attribute__((weak)) double mersenne_array[MT_N];
 static unsigned int th_mt[MT_N];
 
-__attribute__((weak)) void seed_mersenne(long seed)
+static void th_seed(long seed)
 {
   if (seed % 2 == 0) seed++;
   th_mt[0] = (unsigned int)(seed & 0xffffffffL);
@@ -37,9 +38,9 @@ __attribute__((weak)) void seed_mersenne(long seed)
   mersenne_i = 0;
 }
 
-__attribute__((weak)) double mersenne_generate(void)
+static double th_generate(void)
 {
-  if (mersenne_i < 0) seed_mersenne(4357);
+  if (mersenne_i < 0) th_seed(4357);
   for (int k = 0; k < MT_N; k++) {
     unsigned int y = (th_mt[k] & 0x80000000u) | (th_mt[(k + 1) % MT_N] & 0x7fffffffu);
     unsigned int v = th_mt[(k + MT_M) % MT_N] ^ (y >> 1);
@@ -56,6 +57,28 @@ __attribute__((weak)) double mersenne_generate(void)
   }
   mersenne_i = MT_N;
   return mersenne_array[--mersenne_i];
+}
+
+/* weak, so that a program linking mersenne_inline.c uses its own */
+void seed_mersenne(long seed) __attribute__((weak, alias("th_seed")));
+double mersenne_generate(void) __attribute__((weak, alias("th_generate")));
+
+/* Whether the stream is this library's generator (its state words are
+ * then th_mt) or the program's (mersenne_inline.c keeps its words in a
+ * file-static array). */
+static int own_generator(void) { return (void *)mersenne_generate == (void *)th_generate; }
+
+/* The state word behind one output of the last block: the MT tempering is
+ * a bijection of 32-bit words, so x = temper^-1(y). */
+static unsigned int untemper(unsigned int y)
+{
+  y ^= y >> 18;
+  y ^= (y << 15) & 0xefc60000u;
+  unsigned int x = y;
+  for (int k = 0; k < 5; k++) x = y ^ ((x << 7) & 0x9d2c5680u);
+  y = x;
+  for (int k = 0; k < 3; k++) x = y ^ (x >> 11);
+  return x;
 }
 
 static inline double draw(void)
@@ -442,4 +465,77 @@ void update_puregauge_hb(double ***F)
   int rc = th2d_h2d(S.ctx, S.ddraw, S.h0, 2 * vbytes());
   if (!rc) rc = th2d_puregauge_sweep(S.ctx, &S.geo, S.dA, S.ddraw, g, 2);
   if (!fail(rc)) store_links(F);
+}
+
+/* ---- checkpoint of an HMC run --------------------------------------------
+ * The reference cannot resume an HMC run (hmc.c:932-939 loops from a fresh
+ * field).  A checkpoint holds what a resumed run needs to continue the same
+ * Markov chain draw for draw: the field A in FP64, the trajectory counter
+ * and the whole MT19937 state.  Layout (host byte order):
+ *   char magic[8] "TH2DHMC1"; int32 nt, nx, has_words, reserved;
+ *   double m, g, mu; int64 trajectory; double A[nt][nx][2];
+ *   int32 mersenne_i, reserved; double mersenne_array[624];
+ *   uint32 words[624]
+ * The state words are this library's generator state when it supplies the
+ * stream; when the program links its own mersenne_inline.c (whose words
+ * are file-static), they are recovered from the last block by inverting
+ * the tempering (has_words = 2).  Reading restores A, the counter and the
+ * exported block, and the words into this library's generator; with the
+ * program's generator the words cannot be installed, and the read returns
+ * THIRRING_HMC_RNG_PARTIAL. */
+static const char ckpt_magic[8] = {'T', 'H', '2', 'D', 'H', 'M', 'C', '1'};
+
+int thirring_hmc_write_checkpoint(const char *path, double ***F, long trajectory)
+{
+  if (!S.ctx || !path || !F) return TH2D_EARG;
+  FILE *f = fopen(path, "wb");
+  if (!f) return TH2D_EARG;
+  int own = own_generator();
+  int hdr[4] = {S.nt, S.nx, own ? 1 : 2, 0};
+  double par[3] = {m, g, mu};
+  long long traj = trajectory;
+  int st[2] = {mersenne_i, 0};
+  unsigned int words[MT_N];
+  for (int k = 0; k < MT_N; k++)
+    words[k] = own ? th_mt[k] : untemper((unsigned int)(mersenne_array[k] * 4294967296.0));
+  pack_links(F, S.h1);
+  int ok = fwrite(ckpt_magic, 1, 8, f) == 8 && fwrite(hdr, sizeof(int), 4, f) == 4 &&
+           fwrite(par, sizeof(double), 3, f) == 3 && fwrite(&traj, sizeof(traj), 1, f) == 1 &&
+           fwrite(S.h1, sizeof(double), 2 * (size_t)S.V, f) == 2 * (size_t)S.V &&
+           fwrite(st, sizeof(int), 2, f) == 2 &&
+           fwrite(mersenne_array, sizeof(double), MT_N, f) == MT_N &&
+           fwrite(words, sizeof(unsigned int), MT_N, f) == MT_N;
+  ok = (fclose(f) == 0) && ok;
+  return ok ? TH2D_OK : TH2D_EARG;
+}
+
+int thirring_hmc_read_checkpoint(const char *path, double ***F, long *trajectory)
+{
+  if (!S.ctx || !path || !F) return TH2D_EARG;
+  FILE *f = fopen(path, "rb");
+  if (!f) return TH2D_EARG;
+  char magic[8];
+  int hdr[4], st[2];
+  double par[3];
+  long long traj = 0;
+  unsigned int words[MT_N];
+  double arr[MT_N];
+  int ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, ckpt_magic, 8) == 0 &&
+           fread(hdr, sizeof(int), 4, f) == 4 && hdr[0] == S.nt && hdr[1] == S.nx &&
+           fread(par, sizeof(double), 3, f) == 3 && fread(&traj, sizeof(traj), 1, f) == 1 &&
+           fread(S.h1, sizeof(double), 2 * (size_t)S.V, f) == 2 * (size_t)S.V &&
+           fread(st, sizeof(int), 2, f) == 2 && fread(arr, sizeof(double), MT_N, f) == MT_N &&
+           fread(words, sizeof(unsigned int), MT_N, f) == MT_N;
+  fclose(f);
+  if (!ok || st[0] > MT_N) return TH2D_EARG;
+  unpack_links(S.h1, F);
+  m = par[0];
+  g = par[1];
+  mu = par[2];
+  if (trajectory) *trajectory = (long)traj;
+  memcpy(mersenne_array, arr, sizeof(arr));
+  mersenne_i = st[0];
+  if (!own_generator()) return THIRRING_HMC_RNG_PARTIAL;
+  memcpy(th_mt, words, sizeof(words));
+  return TH2D_OK;
 }
