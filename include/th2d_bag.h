@@ -73,6 +73,17 @@ int th2d_lu_logdet(th2d_ctx *ctx, const th2d_batch *b, const void *LU, int preci
 int th2d_lu_inverse(th2d_ctx *ctx, const th2d_batch *b, const void *LU, int precision,
                     const int *piv, void *X);
 
+/* The same inverse without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Its
+ * permutation scratch is a static device array of 4,194,304 ints (16 MiB,
+ * one copy per GPU, held until the process ends): count * max_n must not
+ * exceed that (else TH2D_EARG), and a context must have been created on
+ * the device.  Every call on a GPU shares that array, so calls that
+ * overlap on one GPU race.  Same results as th2d_lu_inverse. */
+int th2d_lu_inverse_noalloc(th2d_ctx *ctx, const th2d_batch *b, const void *LU, int precision,
+                            const int *piv, void *X);
+
 /* C += alpha A B for a batch of jobs (element offsets into one buffer). */
 typedef struct {
   long a, b, c;

@@ -140,6 +140,7 @@ SIGNATURES = {
     "th2d_lu_batched": (I, [P, C.POINTER(Batch), P, I, P, P]),
     "th2d_lu_logdet": (I, [P, C.POINTER(Batch), P, I, P, P, P, P]),
     "th2d_lu_inverse": (I, [P, C.POINTER(Batch), P, I, P, P]),
+    "th2d_lu_inverse_noalloc": (I, [P, C.POINTER(Batch), P, I, P, P]),
     "th2d_gemm_batched": (I, [P, P, I, P, I, I, I, D]),
     "th2d_set_dense_mfma": (I, [I]),
     "th2d_set_lu_panel": (I, [I]),

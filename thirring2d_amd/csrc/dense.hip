@@ -18,6 +18,8 @@
 // trailing update per panel.
 #include "th2d_internal.h"
 
+#include <mutex>
+
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -554,12 +556,12 @@ __global__ void __launch_bounds__(256) k_lu_logdet(const T *base, BatchView bv, 
 // X = P (row permutation of the identity that the pivots define).
 template <typename T>
 __global__ void __launch_bounds__(256) k_perm_identity(T *X, BatchView bv, const int *pivb,
-                                                       int *permb)
+                                                       int *permb, long pstride)
 {
   const int b = blockIdx.y;
   const int n = bv.n[b];
   const int *piv = pivb + bv.poff[b];
-  int *perm = permb + bv.poff[b];
+  int *perm = permb + (pstride > 0 ? pstride * b : bv.poff[b]);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     for (int i = 0; i < n; i++) perm[i] = i;
     for (int i = 0; i < n; i++) {
@@ -572,11 +574,12 @@ __global__ void __launch_bounds__(256) k_perm_identity(T *X, BatchView bv, const
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_fill_perm(T *X, BatchView bv, const int *permb)
+__global__ void __launch_bounds__(256) k_fill_perm(T *X, BatchView bv, const int *permb,
+                                                   long pstride)
 {
   const int b = blockIdx.y;
   const int n = bv.n[b];
-  const int *perm = permb + bv.poff[b];
+  const int *perm = permb + (pstride > 0 ? pstride * b : bv.poff[b]);
   T *Xb = X + bv.off[b];
   long total = (long)n * n;
   for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
@@ -748,29 +751,20 @@ int lu_run(th2d_ctx *c, const th2d_batch *bt, T *A, int *piv, int *info)
   return TH2D_OK;
 }
 
+// The substitutions of the inverse after the permutation scratch is set:
+// perm holds each matrix's permutation at pivot offset poff[b]
+// (pstride 0) or at b * pstride.
 template <typename T>
-int inverse_run(th2d_ctx *c, const th2d_batch *bt, const T *LU, const int *piv, T *X)
+int inverse_launch(th2d_ctx *c, const th2d_batch *bt, const T *LU, const int *piv, T *X,
+                   int *perm, long pstride)
 {
   BatchView bv = view(bt);
   const int N = bt->max_n;
   const bool mf = use_mfma();
-  size_t need = 0;
-  // scratch: permutation per matrix (same offsets as the pivots)
-  {
-    long total = 0;
-    std::vector<long> h(bt->count);
-    TH2D_HIP(hipMemcpyAsync(h.data(), bt->poff, sizeof(long) * bt->count, hipMemcpyDeviceToHost,
-                            c->stream));
-    TH2D_HIP(hipStreamSynchronize(c->stream));
-    for (int i = 0; i < bt->count; i++) total = std::max(total, h[i]);
-    need = sizeof(int) * (size_t)(total + N + 64);
-  }
-  TH2D_TRY(th2d_buf_reserve(c->bag, need));
-  int *perm = (int *)c->bag.ptr;
   hipLaunchKernelGGL(k_perm_identity<T>, dim3(1, bt->count), dim3(256), 0, c->stream, X, bv, piv,
-                     perm);
+                     perm, pstride);
   hipLaunchKernelGGL(k_fill_perm<T>, dim3(std::min(1024, (N * N + 255) / 256), bt->count),
-                     dim3(256), 0, c->stream, X, bv, perm);
+                     dim3(256), 0, c->stream, X, bv, perm, pstride);
   // forward substitution with L (unit lower)
   for (int k = 0; k < N; k += NB) {
     hipLaunchKernelGGL((k_trsm_rows<T, false>), dim3((N + 255) / 256, bt->count), dim3(256), 0,
@@ -801,7 +795,52 @@ int inverse_run(th2d_ctx *c, const th2d_batch *bt, const T *LU, const int *piv, 
   return TH2D_OK;
 }
 
+template <typename T>
+int inverse_run(th2d_ctx *c, const th2d_batch *bt, const T *LU, const int *piv, T *X)
+{
+  // scratch: permutation per matrix (same offsets as the pivots)
+  long total = 0;
+  std::vector<long> h(bt->count);
+  TH2D_HIP(hipMemcpyAsync(h.data(), bt->poff, sizeof(long) * bt->count, hipMemcpyDeviceToHost,
+                          c->stream));
+  TH2D_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < bt->count; i++) total = std::max(total, h[i]);
+  TH2D_TRY(th2d_buf_reserve(c->bag, sizeof(int) * (size_t)(total + bt->max_n + 64)));
+  return inverse_launch<T>(c, bt, LU, piv, X, (int *)c->bag.ptr, 0);
+}
+
 }  // namespace
+
+// ---- the allocation-free inverse ------------------------------------------
+// Its permutation scratch is a file-scope device array: each GPU holds its
+// own copy until the process ends.  Host code uses the per-device address
+// from hipGetSymbolAddress (looked up when a context is created on that
+// device), never the array's name.  Calls that overlap on one GPU share
+// the array and race.
+constexpr long kNoAllocPermInts = 1L << 22;  // 16 MiB: count * max_n <= 4,194,304
+static __device__ __align__(256) int scratch[kNoAllocPermInts];
+static std::mutex g_scratch_mu;
+static int *g_scratch_ptr[64];
+
+namespace th2d {
+int noalloc_scratch_init(int device)
+{
+  if (device < 0 || device >= 64) return TH2D_EARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return TH2D_OK;
+  void *p = nullptr;
+  TH2D_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = (int *)p;
+  return TH2D_OK;
+}
+}  // namespace th2d
+
+static int *noalloc_scratch(int device)
+{
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (device >= 0 && device < 64) ? g_scratch_ptr[device] : nullptr;
+}
+
 
 // ================================================================ ABI ===
 extern "C" int th2d_lu_batched(th2d_ctx *c, const th2d_batch *b, void *A, int precision, int *piv,
@@ -835,6 +874,20 @@ extern "C" int th2d_lu_inverse(th2d_ctx *c, const th2d_batch *b, const void *LU,
   if (!c || !b || !LU || !piv || !X || LU == X || b->count < 1) return TH2D_EARG;
   if (precision == TH2D_PREC_F64) return inverse_run<double>(c, b, (const double *)LU, piv, (double *)X);
   if (precision == TH2D_PREC_F32) return inverse_run<float>(c, b, (const float *)LU, piv, (float *)X);
+  return TH2D_EARG;
+}
+
+extern "C" int th2d_lu_inverse_noalloc(th2d_ctx *c, const th2d_batch *b, const void *LU,
+                                       int precision, const int *piv, void *X)
+{
+  if (!c || !b || !LU || !piv || !X || LU == X || b->count < 1 || b->max_n < 1) return TH2D_EARG;
+  if ((long)b->count * b->max_n > kNoAllocPermInts) return TH2D_EARG;  // beyond the scratch
+  int *perm = noalloc_scratch(c->device);
+  if (!perm) return TH2D_EARG;  // no context was created on this device
+  if (precision == TH2D_PREC_F64)
+    return inverse_launch<double>(c, b, (const double *)LU, piv, (double *)X, perm, b->max_n);
+  if (precision == TH2D_PREC_F32)
+    return inverse_launch<float>(c, b, (const float *)LU, piv, (float *)X, perm, b->max_n);
   return TH2D_EARG;
 }
 

@@ -57,6 +57,7 @@ extern "C" int th2d_ctx_create(th2d_ctx **out, void *stream)
   TH2D_HIP(hipEventCreate(&c->k1));
   TH2D_HIP(hipEventCreate(&c->a0));
   TH2D_HIP(hipEventCreate(&c->a1));
+  TH2D_TRY(th2d::noalloc_scratch_init(c->device));
   *out = c;
   return TH2D_OK;
 }

@@ -331,6 +331,9 @@ int axpy_like(th2d_ctx *ctx, int op, long n, double a, double *y, const double *
               const double *z);
 int reduce_dot(th2d_ctx *ctx, long n_real, const double *a, const double *b, double *dev_out);
 int fetch_doubles(th2d_ctx *ctx, double *host, const double *dev, int n);
+// Device address of the static scratch array of the allocation-free LU
+// inverse on the current device, looked up once per device (dense.hip).
+int noalloc_scratch_init(int device);
 
 // Batched site operations over a device table of nch descriptors covering
 // nblk blocks of 256 sites (table built by bop_finish).
