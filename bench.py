@@ -2,10 +2,14 @@
 FP64, CG true relative residual <= 1e-10), with the Dslash bandwidth.
 
 Workload (one GPU): 256x256 lattice, antiperiodic t and x, m = 0.05,
-g = 0.5, link angles A ~ N(0,1) (device generator, seed 1002), 10 leapfrog
-steps of dt = 0.1 per trajectory, pseudofermion / momentum / stochastic
-noise from the device generator, every CG solve to a true relative
-residual of 1e-10.  One step = one trajectory.
+g = 0.5.  Every chain starts as hmc.c's main does (hmc.c:927-929): A = 0,
+then 100 pure-gauge Metropolis sweeps (device draws), then the warm-up
+trajectories.  A trajectory is 128 leapfrog steps of dt = 1/128 (the
+reference's 10 x 0.1, hmc.c:708-709, is beyond the leapfrog's stability
+limit at this mass and volume: it is timed as a separate, flagged leg),
+with pseudofermion / momentum / stochastic noise from the device generator
+and every CG solve to a true relative residual of 1e-10.  One step = one
+trajectory of every chain; accepted trajectories move the chains.
 
 Prints one JSON line.  `value` is the aggregate trajectories per second of
 a batch of independent 256^2 chains advanced in lockstep (every inversion
@@ -13,8 +17,9 @@ of all chains is one batched solve) with mixed precision (FP32 CG, FP64
 true-residual restarts); the batched FP64 path and single chains in FP64
 and mixed precision are timed beside it.  `roofline` is for the dominant
 kernel, timed with HIP events on its stream: the resident FP32 solver (one
-cooperative launch per inner solve) when the batch fits the device's
-resident workgroups, as 16 chains of 256^2 do, else the CG step kernel.
+cooperative launch per inner solve), whose bound is its FP32 arithmetic
+and per-iteration synchronisation, not HBM (its vectors stay in
+registers).
 """
 from __future__ import annotations
 
@@ -30,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 # measured HBM bytes per launch of the dominant kernel (FETCH_SIZE + WRITE_SIZE,
 # copy-calibrated): k_cg_resident at 16 chains
 # (profiles/k_cg_resident_hmc256_16chains_mixed.md), k_cg_step at 32 chains
@@ -54,6 +60,14 @@ def parse():
                          "(FP64, FP32), then --c5-proposals bag proposals")
     ap.add_argument("--only-headline", action="store_true",
                     help="time the batched mixed-precision leg only (for profiling)")
+    ap.add_argument("--nsteps", type=int, default=128,
+                    help="leapfrog steps per trajectory of the headline (dt = 1/nsteps)")
+    ap.add_argument("--ref-md-steps", type=int, default=3,
+                    help="trajectories of the reference integrator leg, 10 x dt = 0.1 (0 = skip)")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 8-RHS solve leg")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 128^2 force leg")
+    ap.add_argument("--oracle-apply", action="store_true",
+                    help="time the oracle's fm_mul at 256^2 on this host and exit")
     ap.add_argument("--c4-steps", type=int, default=3,
                     help="trajectories per chain of the 256-chain uneven batch (0 = skip)")
     ap.add_argument("--c5-proposals", type=int, default=150,
@@ -98,9 +112,10 @@ def copy_calibration(mib):
           flush=True)
 
 
-def run_batch(H, N, lats, gs, precision, steps, warmup, nsteps=10):
+def run_batch(H, N, lats, gs, precision, steps, warmup, nsteps=10, ctr0=0):
     """Time `steps` trajectories of every chain of `lats` (one batched call
-    per step).  Returns (seconds, per-trajectory records, kernel ms, kernel
+    per step, accept draws from the host: accepted trajectories move the
+    chains).  Returns (seconds, per-trajectory records, kernel ms, kernel
     launches, kernel bytes)."""
     import ctypes as C
     batch = H.ChainBatch(lats)
@@ -109,10 +124,11 @@ def run_batch(H, N, lats, gs, precision, steps, warmup, nsteps=10):
     params = H.cg_params(tol_rel=1e-10, precision=precision, max_iter=200000)
     L = N.lib()
     nch = len(lats)
-    ctr = 0
+    ctr = ctr0
     for _ in range(warmup):
+        u = list(np.random.default_rng(10_000 + ctr).random(nch))
         batch.update_gauge(nsteps=nsteps, traj_len=1.0, params=params, seed=4242,
-                           counters=[ctr] * nch)
+                           counters=[ctr] * nch, u_accept=u)
         ctr += 1
     L.th2d_kernel_timing(batch.ctx.ptr, 1)
     batch.ctx.sync()
@@ -123,8 +139,8 @@ def run_batch(H, N, lats, gs, precision, steps, warmup, nsteps=10):
         rr = batch.update_gauge(nsteps=nsteps, traj_len=1.0, params=params, seed=4242,
                                 counters=[ctr] * nch, u_accept=u)
         ctr += 1
-        res += [(r.dS, r.accepted, r.cg_iterations, r.cg_solves, r.max_true_rel_resid, r.status)
-                for r in rr]
+        res += [(r.dS, r.accepted, r.cg_iterations, r.cg_solves, r.max_true_rel_resid, r.status,
+                 r.acts[1]) for r in rr]
     batch.ctx.sync()
     dt = time.perf_counter() - t0
     ms, n, by = C.c_double(), C.c_long(), C.c_double()
@@ -133,13 +149,52 @@ def run_batch(H, N, lats, gs, precision, steps, warmup, nsteps=10):
     return dt, res, ms.value, n.value, by.value
 
 
-def run_hmc(H, N, size, precision, steps, warmup, chains=1, seed_links=1002, m=0.05, g=0.5):
+def thermalized(H, size, chains, m=0.05, g=0.5, seed=1002):
+    """Chains started as hmc.c:927-929: A = 0, 100 pure-gauge sweeps."""
     lats = []
     for c in range(chains):
         lat = H.StaggeredLattice(size, size, m=m, g=g)
-        lat.random_links(seed_links + c)
+        lat.thermalize_pure_gauge(100, seed=seed + c)
         lats.append(lat)
-    return run_batch(H, N, lats, [g] * chains, precision, steps, warmup)
+    return lats
+
+
+def run_hmc(H, N, size, precision, steps, warmup, chains=1, m=0.05, g=0.5, nsteps=128,
+            lats=None):
+    lats = lats or thermalized(H, size, chains, m, g)
+    return run_batch(H, N, lats, [g] * len(lats), precision, steps, warmup, nsteps=nsteps)
+
+
+def copy_fields(H, lats):
+    """Fresh lattices holding the same fields."""
+    out = []
+    for l in lats:
+        n = H.StaggeredLattice(l.nt, l.nx, m=l.m, g=l.g)
+        n.set_links(l.get_links())
+        out.append(n)
+    return out
+
+
+def mixed_vs_fp64(H, N, lats, nsteps, tau=1e-10):
+    """One trajectory of every chain with mixed and with FP64 solves from
+    the same fields and noise: per-chain |dH_mixed - dH_fp64| against the
+    bound 4 tau |xi|^2 sqrt(m^2+4)/m (two actions, each off by at most
+    |M^-1 xi| |r|)."""
+    out = {}
+    for prec in (N.PREC_MIXED, N.PREC_F64):
+        b = H.ChainBatch(copy_fields(H, lats))
+        out[prec] = b.update_gauge(nsteps=nsteps, params=H.cg_params(tol_rel=tau, precision=prec,
+                                                                     max_iter=200000),
+                                   seed=777, counters=[5] * len(lats),
+                                   u_accept=[0.5] * len(lats))
+    m = lats[0].m
+    d = [abs(a.dS - b.dS) for a, b in zip(out[N.PREC_MIXED], out[N.PREC_F64])]
+    bound = [4 * tau * b.acts[1] * np.sqrt(m * m + 4) / m for b in out[N.PREC_F64]]
+    return {"chains": len(lats), "max_abs_dH_diff": float(max(d)),
+            "max_diff_over_bound": float(max(x / y for x, y in zip(d, bound))),
+            "dH_fp64": [float(r.dS) for r in out[N.PREC_F64]],
+            "dH_mixed": [float(r.dS) for r in out[N.PREC_MIXED]],
+            "bound": "4 tau |xi|^2 sqrt(m^2+4)/m per chain, tau = 1e-10"}
 
 
 def c4_leg(H, N, steps):
@@ -162,10 +217,91 @@ def c4_leg(H, N, steps):
 
 
 def c1_leg(H, N, steps):
-    """16x16 as the reference runs it: m = 0.1, g = 0.5, FP64, 10 x dt = 0.1."""
-    dt, res, *_ = run_hmc(H, N, 16, N.PREC_F64, steps, 2, chains=1, seed_links=1001, m=0.1)
-    return {"value": steps / dt, "unit": "traj/s", "trajectories": steps,
-            "reference_cpu_1core_O3": 23.4, "max_true_rel_resid": max(r[4] for r in res)}
+    """16x16 as the reference runs it (hmc.c:671-746): m = 0.1, g = 0.5,
+    A ~ N(0,1) (seed 1001), FP64, 10 x dt = 0.1; and the stable variant
+    with 40 steps."""
+    out = {}
+    for nsteps, tag in ((10, "nsteps10"), (40, "nsteps40")):
+        lat = H.StaggeredLattice(16, 16, m=0.1, g=0.5)
+        lat.random_links(1001)
+        dt, res, *_ = run_batch(H, N, [lat], [0.5], N.PREC_F64, steps, 2, nsteps=nsteps)
+        out[tag] = {"value": steps / dt, "unit": "traj/s", "trajectories": steps,
+                    "acceptance": float(np.mean([r[1] for r in res])),
+                    "dH_median": float(np.median([r[0] for r in res])),
+                    "max_true_rel_resid": max(r[4] for r in res)}
+    out["reference_cpu_1core_O3_nsteps10"] = 23.4
+    return out
+
+
+def c2_leg(H, N):
+    """256^2, m = 0.05, g = 0.5, A ~ N(0,1) (seed 1002), 8 Gaussian
+    right-hand sides (seed 2002): the 8 solves of (M^dag M) x = b as one
+    batched solve, FP64 and mixed (FP32 inner CG, FP64 true-residual
+    restarts); per RHS the true residual and the iterations."""
+    import ctypes as C
+    size, nrhs = 256, 8
+    V = size * size
+    rng = np.random.default_rng(2002)
+    bs = [rng.normal(size=V) + 1j * rng.normal(size=V) for _ in range(nrhs)]
+    lat = H.StaggeredLattice(size, size, m=0.05, g=0.5)
+    lat.random_links(1002)
+    out = {"lattice": [size, size], "rhs": nrhs, "m": 0.05}
+    L = N.lib()
+    ctx = H.Context.get().ptr
+    bd = [H.DeviceArray(16 * V) for _ in range(nrhs)]
+    xd = [H.DeviceArray(16 * V) for _ in range(nrhs)]
+    for b, d in zip(bs, bd):
+        lat.to_eo(b, d)
+    geoms = (N.Geom * nrhs)(*([lat.geom] * nrhs))
+    xp = (C.c_void_p * nrhs)(*[d.p for d in xd])
+    bp = (C.c_void_p * nrhs)(*[d.p for d in bd])
+    kp = (C.c_void_p * nrhs)(*([lat.K.p] * nrhs))
+    k32 = (C.c_void_p * nrhs)(*([lat.K32.p] * nrhs))
+    for prec, tag in ((N.PREC_F64, "fp64"), (N.PREC_MIXED, "mixed")):
+        p = H.cg_params(tol_rel=1e-10, precision=prec, max_iter=200000)
+        infos = (N.CgInfo * nrhs)()
+        for timed in (0, 1):  # warm-up, then timed
+            if timed:
+                L.th2d_timer_start(ctx)
+            N.check(L.th2d_cg_mdagm_batched(ctx, nrhs, geoms, xp, bp, kp, k32, C.byref(p), infos),
+                    "cg batched")
+        ms = C.c_float()
+        L.th2d_timer_stop(ctx, C.byref(ms))
+        out[tag] = {"ms_8_rhs": ms.value,
+                    "true_rel_resid": [float(i.true_rel_resid) for i in infos],
+                    "iterations": [int(i.iterations) for i in infos],
+                    "status": [int(i.status) for i in infos]}
+    out["mixed_over_fp64_speed"] = out["fp64"]["ms_8_rhs"] / out["mixed"]["ms_8_rhs"]
+    return out
+
+
+def c3_leg(H, N):
+    """Force at 128^2, m = 0.1, g = 0.8 (momentum_step, hmc.c:504-665) on
+    all 32,768 links: the FP64 solve (1e-14) against the mixed one (1e-10),
+    and the device force against the oracle's (CPU, FP64 1e-14)."""
+    import oracle as O
+    nt = nx = 128
+    V = nt * nx
+    A = np.random.default_rng(1003).normal(size=(V, 2))
+    r = np.random.default_rng(2003)
+    xi = r.normal(size=V) + 1j * r.normal(size=V)
+    L = O.lattice(nt, nx, m=0.1, g=0.8)
+    psi = O.fm_dag_mul(L, A, xi)
+    lat = H.StaggeredLattice(nt, nx, m=0.1, g=0.8)
+    lat.set_links(A)
+    chi64, i64 = lat.fmdm_invert_cg(psi, H.cg_params(tol_rel=1e-14))
+    chi32, i32 = lat.fmdm_invert_cg(psi, H.cg_params(tol_rel=1e-10, precision=N.PREC_MIXED))
+    F64, F32 = lat.force(chi64), lat.force(chi32)
+    t0 = time.perf_counter()
+    _, Fo, _ = O.momentum_step(L, np.zeros((V, 2)), A, psi, np.zeros(V, complex), 0.1,
+                               tol_rel=1e-14)
+    t_cpu = time.perf_counter() - t0
+    rel = lambda a, b: float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    return {"links": 2 * V, "fp64_vs_oracle_max_rel": rel(F64, Fo),
+            "mixed_vs_fp64_max_rel": rel(F32, F64),
+            "mixed_bound": 2 * (0.01 + 4) / 0.01 * 1e-10,
+            "cg_iterations_fp64": int(i64.iterations), "cg_iterations_mixed": int(i32.iterations),
+            "oracle_cpu_seconds_1core": t_cpu}
 
 
 def _bag_field(rng, L, rho):
@@ -302,33 +438,63 @@ def dslash_sweep():
           flush=True)
 
 
-def cpu_baseline(iters_per_traj, size):
+# The reference's own fm_mul at 256^2 (-O3, one core) took 5.58 ms per
+# apply in this build's development container; the oracle's restatement
+# took ORACLE_APPLY_MS_DEV on the same host (measured with
+# `python bench.py --oracle-apply`): their ratio carries the reference's
+# speed over to the host this benchmark runs on.
+REF_APPLY_MS_DEV = 5.58
+ORACLE_APPLY_MS_DEV = 7.35
+
+
+def oracle_apply_ms(size=256, reps=20):
+    import oracle as O
+    V = size * size
+    L = O.lattice(size, size, m=0.05, g=0.5)
+    rng = np.random.default_rng(1002)
+    A = rng.normal(size=(V, 2))
+    v = rng.normal(size=V) + 1j * rng.normal(size=V)
+    O.fm_mul(L, A, v)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        O.fm_mul(L, A, v)
+    return 1e3 * (time.perf_counter() - t0) / reps
+
+
+def cpu_baseline(iters_per_traj, size, n_it=1200):
     """The oracle's CG iteration at size^2 on one host core, scaled to a
-    trajectory by the measured GPU iteration count per trajectory."""
+    trajectory by the CG iterations per trajectory of the single-chain FP64
+    run on the GPU."""
     import oracle as O
     V = size * size
     L = O.lattice(size, size, m=0.05, g=0.5)
     rng = np.random.default_rng(1002)
     A = rng.normal(size=(V, 2))
     b = rng.normal(size=V) + 1j * rng.normal(size=V)
-    n_it = 300
     t0 = time.perf_counter()
     O.cg_mdm(L, A, b, tol_abs_rr=0.0, tol_rel=1e-300, max_iter=n_it + 1)
     dt = time.perf_counter() - t0
     per_it = dt / n_it
-    return {"value": 1.0 / (per_it * iters_per_traj), "unit": "traj/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{n_it} CG iterations of the C oracle on {size}^2 "
-                      f"({dt:.1f} s, 1 thread), scaled by {iters_per_traj:.0f} CG iterations "
-                      "per trajectory measured on the GPU run"}
+    out = {"value": 1.0 / (per_it * iters_per_traj), "unit": "traj/s", "cores": 1,
+           "kind": "port",
+           "sample": f"{n_it} CG iterations of the C oracle on {size}^2 "
+                     f"({dt:.1f} s, 1 thread), scaled by {iters_per_traj:.0f} CG iterations "
+                     "per trajectory (single-chain FP64 run on the GPU)"}
+    if ORACLE_APPLY_MS_DEV:
+        out["reference_over_port_apply"] = REF_APPLY_MS_DEV / ORACLE_APPLY_MS_DEV
+    return out
 
 
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
-              "(independent chains and bag matrices shard across GPUs without communication)", file=sys.stderr)
+              "(independent chains and bag matrices shard across GPUs without communication)",
+              file=sys.stderr)
         sys.exit(2)
+    if args.oracle_apply:
+        print(json.dumps({"oracle_fm_mul_ms_256": oracle_apply_ms()}), flush=True)
+        return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
@@ -339,32 +505,59 @@ def main():
         from thirring2d_amd import _native as N
         print(json.dumps({"c5_bag_dense": c5_leg(N, args.c5_proposals)}), flush=True)
         return
+    import ctypes as C
     from thirring2d_amd import _native as N
     from thirring2d_amd import hmc as H
 
-    size, nch = args.size, args.chains
-    # headline: batched mixed precision
+    size, nch, nsteps = args.size, args.chains, args.nsteps
+    start = thermalized(H, size, nch)
+    # headline: batched mixed precision (its warm-up trajectories move the
+    # chains from the pure-gauge start)
+    lats = copy_fields(H, start)
     dt_mx, res_mx, kms, kn, kbytes = run_hmc(H, N, size, N.PREC_MIXED, args.steps, args.warmup,
-                                             chains=nch)
-    import ctypes as C
+                                             nsteps=nsteps, lats=lats)
     on = C.c_int(0)
     N.check(N.lib().th2d_cg_last_resident(H.Context.get().ptr, C.byref(on)), "cg_last_resident")
     resident = bool(on.value)
+    tps = nch * args.steps / dt_mx
+    acc = float(np.mean([r[1] for r in res_mx]))
+    dH = np.array([r[0] for r in res_mx])
+    # the solver kernel: 88 flop per site and iteration (two operator
+    # applications of 34 flop and the fused vector updates), FP32 peak
+    site_iters = kbytes / 120.0
+    tflops = site_iters * 88.0 / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
+    avg_ms = kms / max(kn, 1)
+    traffic = TRAFFIC_BYTES.get((size, nch, resident))
+    roofline = {"bound": "mfma", "achieved": tflops, "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS, "traffic": traffic,
+                "kernel": "k_cg_resident<float>" if resident else "k_cg_step<float>",
+                "launches": kn, "avg_launch_ms": avg_ms,
+                "flop_per_site_iteration": 88,
+                "hbm_GBs_measured": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
+                "note": ("FP32 vector-ALU arithmetic (the FP32 peak, 157.3 TFLOP/s, is the same "
+                         "for the vector ALUs and the matrix cores); the vectors stay in "
+                         "registers and the links in LDS, so HBM sees `traffic` bytes per "
+                         "launch; the rest of each iteration is the synchronisation of a "
+                         "chain's regions" if resident else
+                         "streaming CG: every iteration reads and writes the vectors")}
     if args.only_headline:
-        print(json.dumps({"metric": "batched mixed HMC (profiling run)", "value": nch * args.steps / dt_mx,
-                          "unit": "traj/s", "chains": nch, "kernel_avg_ms": kms / max(kn, 1),
-                          "launches": kn, "resident": resident}), flush=True)
+        print(json.dumps({"metric": "batched mixed HMC (profiling run)", "value": tps,
+                          "unit": "traj/s", "chains": nch, "nsteps": nsteps, "acceptance": acc,
+                          "kernel_avg_ms": avg_ms, "launches": kn, "resident": resident,
+                          "roofline": roofline}), flush=True)
         return
     small = max(2, args.steps // 4)
-    dt_b64, res_b64, *_ = run_hmc(H, N, size, N.PREC_F64, small, 1, chains=nch)
-    dt_1mx, res_1mx, *_ = run_hmc(H, N, size, N.PREC_MIXED, small, 1, chains=1)
-    dt_164, res_164, *_ = run_hmc(H, N, size, N.PREC_F64, small, 1, chains=1)
-    tps = nch * args.steps / dt_mx
+    after = lats  # the chains after the timed trajectories
+    dt_b64, res_b64, *_ = run_hmc(H, N, size, N.PREC_F64, small, 0, nsteps=nsteps,
+                                  lats=copy_fields(H, after))
+    dt_1mx, res_1mx, *_ = run_hmc(H, N, size, N.PREC_MIXED, small, 0, nsteps=nsteps,
+                                  lats=copy_fields(H, after[:1]))
+    dt_164, res_164, *_ = run_hmc(H, N, size, N.PREC_F64, small, 0, nsteps=nsteps,
+                                  lats=copy_fields(H, after[:1]))
     tps_b64 = nch * small / dt_b64
     tps_1mx = small / dt_1mx
     tps_164 = small / dt_164
     its_164 = float(np.mean([r[2] for r in res_164]))
-    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
 
     out = {
         "metric": "HMC traj/s at 256^2 (mixed precision, batched chains, CG true residual <= 1e-10)",
@@ -378,35 +571,38 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
-        "data": "synthetic: A~N(0,1) seeds 1002+chain, device Philox noise",
-        "config": {"workload": f"HMC {size}x{size}, m=0.05, g=0.5, 10 x dt=0.1, CG 1e-10, "
-                               f"{nch} chains per batched step",
-                   "lattice": [size, size], "chains": nch, "nsteps": 10, "dt": 0.1,
-                   "tol_rel": 1e-10},
-        "batched_fp64": {"value": tps_b64, "unit": "traj/s", "chains": nch},
+        "data": "synthetic: A = 0 + 100 pure-gauge sweeps (device Philox, seeds 1002+chain), "
+                "device Philox noise",
+        "config": {"workload": f"HMC {size}x{size}, m=0.05, g=0.5, {nsteps} x dt=1/{nsteps}, "
+                               f"CG 1e-10, {nch} chains per batched step",
+                   "lattice": [size, size], "chains": nch, "nsteps": nsteps,
+                   "dt": 1.0 / nsteps, "tol_rel": 1e-10},
+        "acceptance": acc,
+        "exp_minus_dH_mean": float(np.mean(np.exp(-dH))),
+        "dH_median": float(np.median(dH)),
+        "batched_fp64": {"value": tps_b64, "unit": "traj/s", "chains": nch,
+                         "acceptance": float(np.mean([r[1] for r in res_b64]))},
         "single_chain_mixed": {"value": tps_1mx, "unit": "traj/s"},
         "single_chain_fp64": {"value": tps_164, "unit": "traj/s", "cg_iters_per_traj": its_164},
         "batched_mixed_over_single_fp64": tps / tps_164,
+        "batched_mixed_over_batched_fp64": tps / tps_b64,
+        "single_mixed_over_single_fp64": tps_1mx / tps_164,
         "cg_iters_per_traj": float(np.mean([r[2] for r in res_mx])),
         "max_true_rel_resid": max(r[4] for r in res_mx + res_b64 + res_1mx + res_164),
-        "acceptance": float(np.mean([r[1] for r in res_mx])),
-        "dH_median": float(np.median([r[0] for r in res_mx])),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     # HBM bytes per launch of this kernel at this workload, from the
-                     # FETCH_SIZE / WRITE_SIZE passes in profiles/ (copy-calibrated)
-                     "traffic": TRAFFIC_BYTES.get((size, nch, resident)),
-                     "kernel": ("k_cg_resident<float> (one launch per FP32 inner solve; "
-                                "CG iteration 120 B/site algorithmic)" if resident else
-                                "k_cg_step<float> (CG iteration, 120 B/site algorithmic)"),
-                     "launches": kn, "avg_launch_ms": kms / max(kn, 1),
-                     "note": ("the resident solver keeps each region's vectors in registers "
-                              "and its links in LDS for the whole solve, so its HBM traffic "
-                              "(`traffic`) is far below the streaming CG's algorithmic bytes "
-                              "and frac can exceed 1; it is bound by the per-iteration "
-                              "synchronisation of each chain's regions" if resident else
-                              "streaming: every iteration reads and writes the vectors")},
+        "roofline": roofline,
     }
+    if acc == 0.0:
+        out["flag"] = "acceptance 0: the timed trajectories were all rejected"
+    out["mixed_vs_fp64_dH"] = mixed_vs_fp64(H, N, after, nsteps)
+    if args.ref_md_steps:
+        dt_r, res_r, *_ = run_hmc(H, N, size, N.PREC_MIXED, args.ref_md_steps, 1, nsteps=10,
+                                  lats=copy_fields(H, start))
+        out["reference_md_10x0.1"] = {
+            "value": nch * args.ref_md_steps / dt_r, "unit": "traj/s", "chains": nch,
+            "acceptance": float(np.mean([r[1] for r in res_r])),
+            "dH_median": float(np.median([r[0] for r in res_r])),
+            "flag": "dt = 0.1 (hmc.c:708-709) is beyond the leapfrog stability limit at "
+                    "m = 0.05 on 256^2: every trajectory is rejected"}
     if not args.no_dslash:
         d = {}
         for (sz, nrhs, prec, tag) in [(256, 1, N.PREC_F64, "256_f64"), (256, 8, N.PREC_F64,
@@ -418,6 +614,10 @@ def main():
         out["dslash"] = d
     if args.c1_steps:
         out["c1_16x16_fp64"] = c1_leg(H, N, args.c1_steps)
+    if not args.no_c2:
+        out["c2_256_8rhs"] = c2_leg(H, N)
+    if not args.no_c3:
+        out["c3_force_128"] = c3_leg(H, N)
     if args.c4_steps:
         N.lib().th2d_set_cg_autotune(H.Context.get().ptr, 1)
         out["c4_256_chains_uneven"] = c4_leg(H, N, args.c4_steps)

@@ -16,7 +16,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
-from .hmc import Context, DeviceArray
+from .hmc import _SHUTDOWN, Context, DeviceArray
 
 BC_ANTISYMMETRIC = 0
 BC_OPENX = 1
@@ -176,7 +176,7 @@ class BagState:
 
     def __del__(self):
         try:
-            if self.ptr:
+            if self.ptr and not _SHUTDOWN[0]:
                 N.lib().th2d_bag_destroy(self.ptr)
                 self.ptr = C.c_void_p()
         except Exception:

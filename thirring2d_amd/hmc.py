@@ -10,11 +10,17 @@ runs in the HIP kernels of libth2d.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 
 import numpy as np
 
 from . import _native as N
+
+# Set at interpreter exit, after the context is destroyed: device memory
+# still referenced by live objects is then left to the process's end
+# rather than freed from finalizers while the runtime shuts down.
+_SHUTDOWN = [False]
 
 
 class Context:
@@ -36,6 +42,21 @@ class Context:
         N.check(N.lib().th2d_sync(self.ptr), "th2d_sync")
 
 
+@atexit.register
+def _teardown():
+    """Drain and destroy the context before the interpreter finalizes, so
+    that no HIP call runs during the runtime's own teardown."""
+    ctx = Context._inst
+    if ctx is not None:
+        try:
+            N.lib().th2d_sync(ctx.ptr)
+            N.lib().th2d_ctx_destroy(ctx.ptr)
+        except Exception:
+            pass
+        Context._inst = None
+    _SHUTDOWN[0] = True
+
+
 class DeviceArray:
     """A device allocation with numpy upload/download."""
 
@@ -46,7 +67,7 @@ class DeviceArray:
 
     def __del__(self):
         try:
-            if self.ptr:
+            if self.ptr and not _SHUTDOWN[0]:
                 N.lib().th2d_free(self.ptr)
                 self.ptr = C.c_void_p()
         except Exception:
