@@ -148,6 +148,45 @@ int th2d_bag_accept(th2d_bag *bag, const int *occupy, int n_occupy, const int *f
 /* Device pointer of B and of the field (for tests and batched use). */
 int th2d_bag_views(th2d_bag *bag, double **B, int **field, int *V);
 
+/* ---- many configurations on the device: the update loop of fermionbag.c
+ * (add_link / remove_link, :284-379) with the ratios and accept hooks of
+ * fluctuation_determinant.c:305-871 for a batch of independent bag
+ * matrices of any sizes.  Each matrix keeps B = (D_FF)^{-1}, its field and
+ * log|det D_FF| on the device.  One step proposes one change per matrix
+ * from device (Philox) draws -- with probability 0.3 a monomer on an empty
+ * site or off a monomer (rank 1), otherwise, in direction t or x, a link
+ * on two empty sites, off a link, or a monomer hop onto an empty
+ * neighbour (rank 2) -- forms its ratio det(D_F'F')/det(D_FF) from B and
+ * accepts it when a second draw u < |ratio|; accepted changes are applied
+ * to every matrix in the same launches.  No host synchronisation happens
+ * between steps. */
+typedef struct th2d_bag_batch th2d_bag_batch;
+
+/* One proposal of th2d_bag_batch_sweep: the sites occupied (with their new
+ * field values) and freed (-1 for unused slots), the ratio (0 when the
+ * site had no legal change or the ratio vanished) and the decision. */
+typedef struct {
+  int no, nr;
+  int occ[2], fr[2], vals[2];
+  int accepted, pad;
+  double ratio;
+} th2d_bag_move;
+
+/* count configurations: geoms (host) and their fields concatenated (host
+ * int32, natural order, V_b values each); B of each by batched LU. */
+int th2d_bag_batch_create(th2d_ctx *ctx, int count, const th2d_bag_geom *geoms, const int *fields,
+                          th2d_bag_batch **out);
+int th2d_bag_batch_destroy(th2d_bag_batch *b);
+/* nsteps proposals per matrix; step k draws with counter `counter + k`.
+ * trace: NULL or a device array of nsteps * count records (step-major). */
+int th2d_bag_batch_sweep(th2d_bag_batch *b, int nsteps, unsigned long long seed,
+                         unsigned long long counter, th2d_bag_move *trace);
+/* B, log|det D_FF| and the sign of every matrix again from scratch. */
+int th2d_bag_batch_refresh(th2d_bag_batch *b);
+/* Host copies (any may be NULL): log|det D_FF| and sign per matrix, the
+ * fields (concatenated), the accepted changes per matrix.  Synchronises. */
+int th2d_bag_batch_state(th2d_bag_batch *b, double *logabs, int *sign, int *fields, long *accepted);
+
 /* Free propagator D^{-1} in the reference's eo block layout
  * (calc_Dinv, fermion_matrix.c:191-351): Dinv[e*V/2 + o] = (D^{-1})_{e,o},
  * Dinv[V^2/4 + o*V/2 + e] = (D^{-1})_{o,e} (device, 2 (V/2)^2 doubles),

@@ -324,6 +324,24 @@ def puregauge_sweep(L, rng: Rng, A):
     return A
 
 
+def hmc_chain(nt, nx, m, g, seed, ntraj, nsteps, burn=0, sweeps=100, tol_rel=1e-10):
+    """A Markov chain as hmc.c's main runs it (hmc.c:927-939): A = 0,
+    `sweeps` pure-gauge sweeps, then update_gauge trajectories; returns an
+    (ntraj, 3) array of (accepted, dH, gauge action after the trajectory)
+    for the trajectories after `burn`."""
+    L = lattice(nt, nx, m=m, g=g)
+    rng = Rng(seed, warmup=543210)
+    A = np.zeros((nt * nx, 2))
+    for _ in range(sweeps):
+        A = puregauge_sweep(L, rng, A)
+    out = []
+    for k in range(burn + ntraj):
+        acc, dS, _, _, A = update_gauge(L, rng, A, nsteps=nsteps, tol_rel=tol_rel)
+        if k >= burn:
+            out.append((acc, dS, gauge_action(L, A)))
+    return np.array(out)
+
+
 # ---------------------------------------------------------------- bag ---
 def bag_lattice(nt, nx, bc=BC_ANTISYMMETRIC, with_mass_monomers=0, m=0.1, mu=0.0):
     return BagLattice(nt, nx, bc, with_mass_monomers, m, mu)

@@ -63,6 +63,11 @@ class BagGeom(C.Structure):
                 ("mu", D)]
 
 
+class BagMove(C.Structure):
+    _fields_ = [("no", I), ("nr", I), ("occ", I * 2), ("fr", I * 2), ("vals", I * 2),
+                ("accepted", I), ("pad", I), ("ratio", D)]
+
+
 class Batch(C.Structure):
     _fields_ = [("count", I), ("max_n", I), ("off", P), ("n", P), ("poff", P)]
 
@@ -159,6 +164,11 @@ SIGNATURES = {
     "th2d_bag_brute_enumerate": (I, [P, C.POINTER(BagGeom), P, P, C.POINTER(L)]),
     "th2d_bag_matrix_element": (I, [C.POINTER(BagGeom), I, I, I, I, D, C.POINTER(D)]),
     "th2d_bag_determinant": (I, [P, C.POINTER(BagGeom), P, C.POINTER(D), C.POINTER(I)]),
+    "th2d_bag_batch_create": (I, [P, I, P, P, C.POINTER(P)]),
+    "th2d_bag_batch_destroy": (I, [P]),
+    "th2d_bag_batch_sweep": (I, [P, I, U64, U64, P]),
+    "th2d_bag_batch_refresh": (I, [P]),
+    "th2d_bag_batch_state": (I, [P, P, P, P, P]),
 }
 
 _lib = None

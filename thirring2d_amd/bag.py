@@ -212,6 +212,62 @@ class BagState:
                 "th2d_bag_accept")
 
 
+class BagBatch:
+    """Many bag configurations advanced by Metropolis proposals in lockstep
+    on the device (th2d_bag_batch_*): the update loop of fermionbag.c:
+    284-379 over the determinant ratios of fluctuation_determinant.c for a
+    batch of matrices of any sizes."""
+
+    def __init__(self, lattices, fields):
+        self.ctx = Context.get()
+        self.lats = list(lattices)
+        self.count = len(self.lats)
+        self.Vs = [l.V for l in self.lats]
+        geoms = (N.BagGeom * self.count)(*[l.geom for l in self.lats])
+        flat = np.concatenate([_i32(f) for f in fields])
+        self.ptr = C.c_void_p()
+        N.check(N.lib().th2d_bag_batch_create(self.ctx.ptr, self.count, geoms,
+                                              flat.ctypes.data_as(C.c_void_p),
+                                              C.byref(self.ptr)), "th2d_bag_batch_create")
+
+    def __del__(self):
+        try:
+            if self.ptr and not _SHUTDOWN[0]:
+                N.lib().th2d_bag_batch_destroy(self.ptr)
+                self.ptr = C.c_void_p()
+        except Exception:
+            pass
+
+    def sweep(self, nsteps, seed=0, counter=0, trace=False):
+        """nsteps proposals per matrix; with trace, the proposals as an
+        (nsteps, count) array of BagMove records."""
+        tr = DeviceArray(C.sizeof(N.BagMove) * nsteps * self.count) if trace else None
+        N.check(N.lib().th2d_bag_batch_sweep(self.ptr, nsteps, seed, counter,
+                                             tr.p if tr else None), "th2d_bag_batch_sweep")
+        if not trace:
+            return None
+        raw = tr.download((C.sizeof(N.BagMove) * nsteps * self.count,), np.uint8)
+        recs = (N.BagMove * (nsteps * self.count)).from_buffer_copy(raw.tobytes())
+        return [list(recs[k * self.count:(k + 1) * self.count]) for k in range(nsteps)]
+
+    def refresh(self):
+        N.check(N.lib().th2d_bag_batch_refresh(self.ptr), "th2d_bag_batch_refresh")
+
+    def state(self):
+        """(log|det D_FF|, sign, fields, accepted) per matrix."""
+        la = np.empty(self.count)
+        sg = np.empty(self.count, np.int32)
+        fl = np.empty(sum(self.Vs), np.int32)
+        acc = np.empty(self.count, np.int64)
+        N.check(N.lib().th2d_bag_batch_state(self.ptr, la.ctypes.data_as(C.c_void_p),
+                                             sg.ctypes.data_as(C.c_void_p),
+                                             fl.ctypes.data_as(C.c_void_p),
+                                             acc.ctypes.data_as(C.c_void_p)),
+                "th2d_bag_batch_state")
+        off = np.cumsum([0] + self.Vs)
+        return la, sg, [fl[off[k]:off[k + 1]] for k in range(self.count)], acc
+
+
 def build_masked_batch(lattices, fields, precision=N.PREC_F64) -> DenseBatch:
     """Dense masked matrices (identity rows/columns at occupied sites,
     fM_index elements between free sites; vec_ops.c:107-131) of several

@@ -518,28 +518,6 @@ int axpy_like(th2d_ctx *c, int op, long n, double a, double *y, const double *x,
 }  // namespace th2d
 
 // ============================================================ RNG =======
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), keyed by a 64-bit seed
-// and a 128-bit counter; uniform doubles in (0,1), never 0.
-__device__ __forceinline__ uint4 philox10(uint4 c, uint2 k)
-{
-  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-#pragma unroll
-  for (int r = 0; r < 10; r++) {
-    uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
-    uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
-    c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
-    k.x += W0;
-    k.y += W1;
-  }
-  return c;
-}
-
-__device__ __forceinline__ double u01(uint32_t a, uint32_t b)
-{
-  uint64_t v = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);  // 53 bits
-  return ((double)v + 0.5) * (1.0 / 9007199254740992.0);
-}
-
 // Two independent N(0,1) for (seed, stream, index).
 __device__ __forceinline__ double2 gauss2(unsigned long long seed, uint32_t stream, uint32_t idx,
                                           uint32_t ctr)
