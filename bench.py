@@ -325,7 +325,7 @@ def _bag_field(rng, L, rho):
     return f
 
 
-def c5_leg(N, proposals):
+def c5_leg(N, proposals, device_steps=1000):
     """64 masked bag matrices, L uniform on even {16..48} (seed 5005),
     occupancy rho ~ U[0.1, 0.5], m = 0.1: batched LU log-det in FP64 and
     FP32 on MFMA, then rank-1/rank-2 ratio+update proposals."""
@@ -354,6 +354,21 @@ def c5_leg(N, proposals):
         la[tag], sg = db.logdet()
         tf = flops / (ms.value * 1e-3) / 1e12
         out["lu_" + tag] = {"ms": ms.value, "TFLOP/s": tf, "peak": peak, "mfma_frac": tf / peak}
+    # the FP32 factors with the FP64 correction of their log-determinant
+    db64 = Bm.build_masked_batch(lats, fields, N.PREC_F64)
+    db32 = Bm.build_masked_batch(lats, fields, N.PREC_F32)
+    db32.lu()
+    db32.logdet_refined(db64)  # warm-up
+    ctx.sync()
+    lib.th2d_timer_start(ctx.ptr)
+    lar, _ = db32.logdet_refined(db64)
+    ms = C.c_float()
+    lib.th2d_timer_stop(ctx.ptr, C.byref(ms))
+    out["lu_f32_refined"] = {"correction_ms": ms.value,
+                             "total_ms": out["lu_f32"]["ms"] + ms.value,
+                             "logdet_max_abs_err_vs_f64": float(np.max(np.abs(lar - la["f64"]))),
+                             "logdet_max_rel_err_vs_f64": float(np.max(np.abs(lar - la["f64"]) /
+                                                                       np.abs(la["f64"])))}
     out["f32_logdet_max_abs_err_vs_f64"] = float(np.max(np.abs(la["f32"] - la["f64"])))
     out["f32_logdet_max_rel_err_vs_f64"] = float(np.max(np.abs(la["f32"] - la["f64"]) /
                                                         np.abs(la["f64"])))
@@ -389,10 +404,36 @@ def c5_leg(N, proposals):
                 n_acc += 1
         t_up += time.perf_counter() - t0
         del st
-    out["woodbury"] = {"proposals": n_up, "accepted": n_acc, "seconds": t_up,
-                       "proposals_per_s": n_up / t_up if t_up > 0 else 0.0,
-                       "matrices": 8}
+    out["woodbury_host_driven"] = {"proposals": n_up, "accepted": n_acc, "seconds": t_up,
+                                   "proposals_per_s": n_up / t_up if t_up > 0 else 0.0,
+                                   "matrices": 8}
+    out["device_driver"] = bag_driver_leg(N, lats, fields, device_steps)
     return out
+
+
+def bag_driver_leg(N, lats, fields, steps):
+    """All 64 matrices through `steps` proposals each on the device
+    (th2d_bag_batch_sweep: one proposal per matrix per step, no host
+    synchronisation between steps), then the drift of log|det D_FF| against
+    a refresh from scratch."""
+    from thirring2d_amd import bag as Bm
+    import ctypes as C
+    batch = Bm.BagBatch(lats, fields)
+    batch.sweep(20, seed=5, counter=0)  # warm-up
+    _, _, _, acc0 = batch.state()
+    lib = N.lib()
+    lib.th2d_timer_start(batch.ctx.ptr)
+    batch.sweep(steps, seed=5, counter=1000)
+    ms = C.c_float()
+    lib.th2d_timer_stop(batch.ctx.ptr, C.byref(ms))
+    la, sg, _, acc = batch.state()
+    batch.refresh()
+    la2, sg2, _, _ = batch.state()
+    n = len(lats) * steps
+    return {"matrices": len(lats), "proposals": n, "accepted": int(np.sum(acc - acc0)),
+            "ms": ms.value, "proposals_per_s": n / (ms.value * 1e-3),
+            "logdet_drift_max_abs": float(np.max(np.abs(la - la2))),
+            "sign_changes_on_refresh": int(np.sum(sg != sg2))}
 
 
 def dslash_bw(H, N, size, nrhs, precision, reps=50):

@@ -69,6 +69,15 @@ int th2d_lu_batched(th2d_ctx *ctx, const th2d_batch *b, void *A, int precision, 
  * 171-175, accumulated as logs). */
 int th2d_lu_logdet(th2d_ctx *ctx, const th2d_batch *b, const void *LU, int precision,
                    const int *piv, const int *info, double *logabs, int *sign);
+/* log|det A| of the FP32 factors of A (th2d_lu_batched, TH2D_PREC_F32),
+ * corrected in FP64 (determinant_mu, full_determinant.c:144-186, with the
+ * factorisation in FP32 on the matrix cores): with B = P L U the product of
+ * the FP32 factors and E = A - B formed in FP64,
+ *   log|det A| = log|det B| + tr(B^-1 E) + O(|B^-1 E|^2),
+ * where tr(B^-1 E) takes B^-1 from the FP32 factors.  A holds the FP64
+ * matrices (device, batch layout, unchanged); sign as th2d_lu_logdet. */
+int th2d_lu_logdet_refined(th2d_ctx *ctx, const th2d_batch *b, const double *A, const float *LU32,
+                           const int *piv, const int *info, double *logabs, int *sign);
 /* X = A^{-1} from the factors (dgetri semantics); X uses the batch layout. */
 int th2d_lu_inverse(th2d_ctx *ctx, const th2d_batch *b, const void *LU, int precision,
                     const int *piv, void *X);

@@ -144,6 +144,7 @@ SIGNATURES = {
     "th2d_bag_cg": (I, [P, C.POINTER(BagGeom), P, P, P, I, D, I, C.POINTER(I), C.POINTER(I)]),
     "th2d_lu_batched": (I, [P, C.POINTER(Batch), P, I, P, P]),
     "th2d_lu_logdet": (I, [P, C.POINTER(Batch), P, I, P, P, P, P]),
+    "th2d_lu_logdet_refined": (I, [P, C.POINTER(Batch), P, P, P, P, P, P]),
     "th2d_lu_inverse": (I, [P, C.POINTER(Batch), P, I, P, P]),
     "th2d_lu_inverse_noalloc": (I, [P, C.POINTER(Batch), P, I, P, P]),
     "th2d_gemm_batched": (I, [P, P, I, P, I, I, I, D]),

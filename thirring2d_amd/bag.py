@@ -148,6 +148,16 @@ class DenseBatch:
                                        self.piv.p, self.info.p, la.p, sg.p), "th2d_lu_logdet")
         return la.download((self.count,), np.float64), sg.download((self.count,), np.int32)
 
+    def logdet_refined(self, original: "DenseBatch"):
+        """log|det| of these FP32 factors corrected in FP64 against the FP64
+        matrices `original` (th2d_lu_logdet_refined)."""
+        la = DeviceArray(8 * self.count)
+        sg = DeviceArray(4 * self.count)
+        N.check(N.lib().th2d_lu_logdet_refined(self.ctx.ptr, C.byref(self.batch), original.A.p,
+                                               self.A.p, self.piv.p, self.info.p, la.p, sg.p),
+                "th2d_lu_logdet_refined")
+        return la.download((self.count,), np.float64), sg.download((self.count,), np.int32)
+
     def inverse(self):
         X = DeviceArray(self.A.nbytes)
         N.check(N.lib().th2d_lu_inverse(self.ctx.ptr, C.byref(self.batch), self.A.p, self.prec,

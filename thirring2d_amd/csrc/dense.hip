@@ -666,6 +666,54 @@ __global__ void k_f64_to_f32_n(float *d, const double *s, long n)
     d[i] = (float)s[i];
 }
 
+// ---- FP64 correction of an FP32 LU's log-determinant ---------------------
+// The promoted factors: L (unit lower) and U (upper) as dense FP64
+// matrices, and G = 0, all in the batch layout.
+__global__ void __launch_bounds__(256) k_expand_factors(const float *LU32, BatchView bv, double *L,
+                                                        double *U, double *G)
+{
+  const int b = blockIdx.y;
+  const long n = bv.n[b], off = bv.off[b];
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n * n; e += (long)gridDim.x * 256) {
+    const long i = e / n, j = e - i * n;
+    const double v = (double)LU32[off + e];
+    L[off + e] = i > j ? v : (i == j ? 1.0 : 0.0);
+    U[off + e] = i <= j ? v : 0.0;
+    G[off + e] = 0.0;
+  }
+}
+
+// Partial sums of tr(X E) = sum_{r,i} X[i][perm[r]] (A[perm[r]][i] - G[r][i])
+// with E = A - P L U the FP64 residual of the FP32 factorisation (rows of
+// A permuted as the factorisation permuted them: row r of P^T A is
+// A[perm[r]]) and X the FP32 inverse of P L U.
+__global__ void __launch_bounds__(256) k_trace_correction(const double *A, const float *X,
+                                                          const double *G, const int *permb,
+                                                          BatchView bv, double *part, int nparts)
+{
+  __shared__ double sh[4];
+  const int b = blockIdx.y;
+  const long n = bv.n[b], off = bv.off[b];
+  const int *perm = permb + bv.poff[b];
+  double acc = 0.0;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < n * n; e += (long)gridDim.x * 256) {
+    const long r = e / n, i = e - r * n, pr = perm[r];
+    acc += (double)X[off + i * n + pr] * (A[off + pr * n + i] - G[off + e]);
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[(long)b * nparts + blockIdx.x] = s;
+}
+
+__global__ void k_add_correction(const double *part, int nparts, int count, const int *info,
+                                 double *logabs)
+{
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= count || info[b]) return;
+  double s = 0.0;
+  for (int q = 0; q < nparts; q++) s += part[(long)b * nparts + q];
+  logabs[b] += s;
+}
+
 static int g_use_mfma = -1;
 static int g_lu_reg_panel = 1;  // register panel when it fits (th2d_set_lu_panel)
 bool use_mfma()
@@ -864,6 +912,85 @@ extern "C" int th2d_lu_logdet(th2d_ctx *c, const th2d_batch *b, const void *LU, 
     hipLaunchKernelGGL(k_lu_logdet<float>, dim3(b->count), dim3(256), 0, c->stream,
                        (const float *)LU, bv, piv, info, logabs, sign);
   else return TH2D_EARG;
+  TH2D_HIP(hipGetLastError());
+  return TH2D_OK;
+}
+
+// log|det A| of FP32 factors, corrected in FP64: log det A = log det B +
+// tr(B^-1 E) + O(|B^-1 E|^2) with B = P L U (the FP32 factors) and
+// E = A - B formed in FP64 from the promoted factors (a triangle-aware
+// MFMA product: the tile (I, J) of L U needs depth min(I, J) + 64 only)
+// and B^-1 the FP32 inverse from the same factors.
+extern "C" int th2d_lu_logdet_refined(th2d_ctx *c, const th2d_batch *b, const double *A,
+                                      const float *LU32, const int *piv, const int *info,
+                                      double *logabs, int *sign)
+{
+  if (!c || !b || !A || !LU32 || !piv || !info || !logabs || !sign || b->count < 1)
+    return TH2D_EARG;
+  TH2D_TRY(th2d_lu_logdet(c, b, LU32, TH2D_PREC_F32, piv, info, logabs, sign));
+  const int count = b->count;
+  std::vector<long> off(count), poff(count);
+  std::vector<int> n(count);
+  TH2D_HIP(hipMemcpyAsync(off.data(), b->off, sizeof(long) * count, hipMemcpyDeviceToHost, c->stream));
+  TH2D_HIP(hipMemcpyAsync(poff.data(), b->poff, sizeof(long) * count, hipMemcpyDeviceToHost,
+                          c->stream));
+  TH2D_HIP(hipMemcpyAsync(n.data(), b->n, sizeof(int) * count, hipMemcpyDeviceToHost, c->stream));
+  TH2D_HIP(hipStreamSynchronize(c->stream));
+  long total = 0, ptotal = 0;
+  for (int q = 0; q < count; q++) {
+    total = std::max(total, off[q] + (long)n[q] * n[q]);
+    ptotal = std::max(ptotal, poff[q] + (long)n[q]);
+  }
+  // per-tile GEMM jobs of G = L U
+  std::vector<th2d_gemm_job> jobs;
+  for (int q = 0; q < count; q++)
+    for (int I = 0; I < n[q]; I += BM)
+      for (int J = 0; J < n[q]; J += BN) {
+        th2d_gemm_job j;
+        j.M = std::min(BM, n[q] - I);
+        j.N = std::min(BN, n[q] - J);
+        j.K = std::min(n[q], std::min(I, J) + BM);
+        j.a = total + off[q] + (long)I * n[q];             // L rows I..
+        j.b = 2 * total + off[q] + J;                        // U columns J..
+        j.c = off[q] + (long)I * n[q] + J;                   // G
+        j.lda = j.ldb = j.ldc = n[q];
+        jobs.push_back(j);
+      }
+  const int nparts = 64;
+  // scratch: G, L, U (FP64, `total` each), X (FP32), jobs, perm, partials
+  const size_t bytes = sizeof(double) * 3 * (size_t)total + sizeof(float) * (size_t)total +
+                       sizeof(th2d_gemm_job) * jobs.size() + sizeof(int) * (size_t)(ptotal + 64) +
+                       sizeof(double) * (size_t)count * nparts + 1024;
+  TH2D_TRY(th2d_buf_reserve(c->misc, bytes));
+  char *p = (char *)c->misc.ptr;
+  double *G = (double *)p;
+  double *L = G + total, *U = L + total;
+  float *X = (float *)(U + total);
+  th2d_gemm_job *djobs = (th2d_gemm_job *)(((uintptr_t)(X + total) + 255) & ~(uintptr_t)255);
+  int *perm = (int *)(djobs + jobs.size());
+  double *part = (double *)(((uintptr_t)(perm + ptotal + 64) + 255) & ~(uintptr_t)255);
+  BatchView bv = view(b);
+  const int N = b->max_n;
+  TH2D_HIP(hipMemcpyAsync(djobs, jobs.data(), sizeof(th2d_gemm_job) * jobs.size(),
+                          hipMemcpyHostToDevice, c->stream));
+  TH2D_TRY(inverse_launch<float>(c, b, LU32, piv, X, perm, 0));
+  hipLaunchKernelGGL(k_expand_factors, dim3(std::min(2048, (N * N + 255) / 256), count), dim3(256),
+                     0, c->stream, LU32, bv, L, U, G);
+  const bool mf = use_mfma();
+  for (size_t j0 = 0; j0 < jobs.size(); j0 += 65535) {
+    const unsigned nj = (unsigned)std::min<size_t>(65535, jobs.size() - j0);
+    if (mf)
+      hipLaunchKernelGGL((k_gemm_jobs<double, true>), dim3(1, 1, nj), dim3(256), 0, c->stream, G,
+                         djobs + j0, 1.0);
+    else
+      hipLaunchKernelGGL((k_gemm_jobs<double, false>), dim3(1, 1, nj), dim3(256), 0, c->stream, G,
+                         djobs + j0, 1.0);
+  }
+  // perm (inverse_launch left it there) pairs each row of L U with its row of A
+  hipLaunchKernelGGL(k_trace_correction, dim3(nparts, count), dim3(256), 0, c->stream, A, X, G,
+                     perm, bv, part, nparts);
+  hipLaunchKernelGGL(k_add_correction, dim3((count + 63) / 64), dim3(64), 0, c->stream, part,
+                     nparts, count, info, logabs);
   TH2D_HIP(hipGetLastError());
   return TH2D_OK;
 }
