@@ -71,6 +71,22 @@ void removed_monomer(int t1, int x1, int nu);                           /* :435-
 void moved_source_monomer(int t2, int x2, int t4, int x4);              /* :484-524   */
 void not_moved_source_monomer(int t2, int x2, int t4, int x4);          /* :526-545   */
 
+/* ---- fermionbag.c measurement over these ratios and hooks -------------- */
+/* The worm estimate of the susceptibility (fermionbag.c:450-590) on the
+ * program's field, drawing from the MT stream (mersenne.h) in the
+ * reference's order: returns the summed signs (`steps`) and, in *value,
+ * steps * n_links / (4 n_attempts V), the number measure_susceptibility
+ * prints.  Antisymmetric lattices only. */
+long thirring_bag_worm(int n_attempts, double *value);
+/* 20 worms, printing "Susceptibility %g" (weak: a program's own
+ * measure_susceptibility takes precedence). */
+void measure_susceptibility(void);
+/* reference RNG interface (mersenne.h), weak here as in libthirring_hmc */
+extern int mersenne_i;
+extern double mersenne_array[624];
+void seed_mersenne(long seed);
+double mersenne_generate(void);
+
 #ifdef __cplusplus
 }
 #endif

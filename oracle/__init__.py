@@ -86,6 +86,7 @@ def lib():
             "orc_calc_Dinv": (I, [P, P]),
             "orc_bag_weight": (I, [P, P, P, P, P, P]),
             "orc_brute_enumerate": (C.c_long, [P, P, P]),
+            "orc_worm": (C.c_long, [P, P, P, D, I, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -423,3 +424,14 @@ def brute_enumerate(B):
     Z = np.zeros((V // 2 + 1, V + 1)); Za = np.zeros_like(Z)
     n = lib().orc_brute_enumerate(C.byref(B), _p(Z), _p(Za))
     return Z, Za, n
+
+
+def worm(B, field, rng: Rng, U, n_attempts=20, n_links=None, n_monomers=0, sign=1):
+    """The worm susceptibility (fermionbag.c:450-590); returns (steps, value,
+    field, n_links, n_monomers, sign)."""
+    f = np.array(field, dtype=np.int32)
+    nl = I(int(np.count_nonzero((f == 2) | (f == 3))) if n_links is None else n_links)
+    nm, sg, val = I(n_monomers), I(sign), D()
+    steps = lib().orc_worm(C.byref(B), _p(f), rng.ref, U, n_attempts, C.byref(nl), C.byref(nm),
+                           C.byref(sg), C.byref(val))
+    return steps, val.value, f, nl.value, nm.value, sg.value

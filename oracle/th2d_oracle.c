@@ -901,3 +901,148 @@ long orc_brute_enumerate(const orc_bag_lattice *B, double *Z, double *Zabs)
   free(c.Dinv);
   return c.visited;
 }
+
+/* ------------------------------------------------------------------ */
+/* Worm estimate of the susceptibility (fermionbag.c:450-590)          */
+/* ------------------------------------------------------------------ */
+enum { W_LINK_TUP = 2, W_SOURCE = 6, W_NDIRS = 4 };
+
+static int w_nb(const orc_bag_lattice *B, int s, int dir)
+{
+  int t = s / B->nx, x = s % B->nx;
+  if (dir == 0) t = (t + 1) % B->nt;
+  else if (dir == 2) t = (t - 1 + B->nt) % B->nt;
+  else if (dir == 1) x = (x + 1) % B->nx;
+  else x = (x - 1 + B->nx) % B->nx;
+  return t * B->nx + x;
+}
+
+/* links in the order update_linklists lists them (fermionbag.c:166-178):
+ * tails (values 2, 3) in site order, as dir * V + site */
+static int w_links(const orc_bag_lattice *B, const int *f, int *links)
+{
+  int V = B->nt * B->nx, n = 0;
+  for (int s = 0; s < V; s++)
+    if (f[s] >= W_LINK_TUP && f[s] < W_LINK_TUP + 2) links[n++] = (f[s] - W_LINK_TUP) * V + s;
+  return n;
+}
+
+static void w_link_on(const orc_bag_lattice *B, int *f, int s, int dir)
+{
+  int s2 = w_nb(B, s, dir);
+  f[s] = dir + 2;
+  f[s2] = (dir + 2) % 4 + 2;
+}
+
+/* |det D_F'F' / det D_FF| with F' = F minus occ plus fr (natural sites) */
+static double w_ratio(const orc_bag_lattice *B, int *f, const int *occ, int no, const int *fr, int nf,
+                      int *new_sign)
+{
+  int V = B->nt * B->nx;
+  int *g = malloc(sizeof(int) * V);
+  memcpy(g, f, sizeof(int) * V);
+  for (int a = 0; a < nf; a++) g[fr[a]] = 0;
+  for (int a = 0; a < no; a++) g[occ[a]] = 1;
+  double l0, l1;
+  int s0, s1;
+  orc_determinant_mu(B, f, B->mu, &l0, &s0);
+  orc_determinant_mu(B, g, B->mu, &l1, &s1);
+  free(g);
+  if (new_sign) *new_sign = s1;
+  return s1 == 0 ? 0.0 : exp(l1 - l0);
+}
+
+long orc_worm(const orc_bag_lattice *B, int *f, orc_mt *st, double U, int n_attempts,
+              int *n_links, int *n_monomers, int *sign, double *value)
+{
+  int V = B->nt * B->nx;
+  double ld;
+  int sd;
+  int *empty = calloc(V, sizeof(int));
+  orc_determinant_mu(B, empty, B->mu, &ld, &sd);
+  free(empty);
+  int *links = malloc(sizeof(int) * 2 * V);
+  long steps = 0;
+  double scale = (double)*n_links / (W_NDIRS * n_attempts * V);
+  *n_links = w_links(B, f, links);
+  if (*n_links > 0)
+    for (int attempt = 0; attempt < n_attempts; attempt++) {
+      int l = links[(int)(orc_mt_next(st) * *n_links)];
+      int s1 = l % V, nu = l / V;
+      if (orc_mt_next(st) > 0.5) s1 = w_nb(B, s1, nu);
+      int dir = f[s1] - W_LINK_TUP;
+      int s2 = w_nb(B, s1, dir);
+      f[s1] = W_SOURCE;
+      f[s2] = W_SOURCE;
+      *n_monomers += 2;
+      *n_links -= 1;
+      steps += *sign;
+      for (;;) {
+        int d = (int)(W_NDIRS * orc_mt_next(st));
+        int s3 = w_nb(B, s2, d);
+        if (orc_mt_next(st) > 0.5) {
+          if (s3 == s1) {  /* back at the start: the link again */
+            f[s1] = 0;
+            f[s2] = 0;
+            w_link_on(B, f, s2, d);
+            *n_monomers -= 2;
+            *n_links += 1;
+            break;
+          } else if (f[s3] >= W_LINK_TUP) {  /* flip the link */
+            int ld3 = f[s3] - W_LINK_TUP;
+            int s4 = w_nb(B, s3, ld3);
+            f[s2] = 0;
+            f[s3] = 0;
+            f[s4] = 0;
+            w_link_on(B, f, s2, d);
+            f[s4] = W_SOURCE;
+            s2 = s4;
+          } else if (f[s3] == 1) {  /* swap with the monomer */
+            f[s2] = 1;
+            f[s3] = W_SOURCE;
+            s2 = s3;
+          }
+        } else if (B->with_mass_monomers) {
+          int d2;
+          do d2 = (int)(W_NDIRS * orc_mt_next(st));
+          while (d2 == (d + 2) % 4);
+          int s4 = w_nb(B, s3, d2);
+          (void)orc_mt_next(st); /* the hop of :532-544 is never taken */
+          if (f[s3] == 0 && f[s4] == 0) {  /* a new link (s3, s4), the source to s4 */
+            int occ[2] = {s3, s4}, ns;
+            double p = U * w_ratio(B, f, occ, 2, NULL, 0, &ns);
+            if (orc_mt_next(st) < p) {
+              *sign = ns * sd;  /* new_link commits the occupation of s3, s4 */
+              f[s2] = 0;
+              f[s3] = 0;
+              f[s4] = 0;
+              w_link_on(B, f, s2, d);
+              f[s4] = W_SOURCE;
+              s2 = s4;
+            }
+          } else if (f[s3] == W_LINK_TUP + d2) {  /* remove the link (s3, s4) */
+            int fr[2] = {s2, s3}, ns;
+            double p = w_ratio(B, f, NULL, 0, fr, 2, &ns) / U;
+            if (orc_mt_next(st) < p) {
+              f[s2] = 0;
+              f[s3] = 0;
+              f[s4] = 0;
+              f[s4] = W_SOURCE;  /* the link (s2, s3) goes on and off again */
+              *n_links -= 1;
+              *sign = ns * sd;
+              s2 = s4;
+            }
+          }
+        } else {
+          /* the hop to an empty neighbour (:565-577): the neighbour has the
+           * other parity, so the ratio is 0 and the move is rejected */
+          if (f[s3] == 0) (void)orc_mt_next(st);
+        }
+        steps += *sign;
+      }
+      *n_links = w_links(B, f, links);
+    }
+  free(links);
+  *value = (double)steps * scale;
+  return steps;
+}

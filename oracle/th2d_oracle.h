@@ -207,6 +207,24 @@ int orc_bag_weight(const orc_bag_lattice *B, const int *field, const double *Din
  * number of configurations visited. */
 long orc_brute_enumerate(const orc_bag_lattice *B, double *Z, double *Zabs);
 
+/* The worm estimate of the susceptibility (measure_susceptibility,
+ * fermionbag.c:450-590) on the antisymmetric lattice: n_attempts worms,
+ * each opening a random link into two source monomers and moving one of
+ * them until it closes; field (natural order, int32) is updated in place,
+ * n_links / n_monomers / sign carry the reference's globals (sign is
+ * current_sign: the sign of det(D_FF) det(D)), and the draws come from the
+ * MT stream in the reference's order.  With mass monomers the live moves
+ * are the link flip, the monomer swap, a new link (det_added_link, weight U)
+ * and a link removal (det_removed_link / U); without, the hop branch only
+ * draws (its ratio vanishes: the neighbour has the other parity).  Ratios
+ * come from orc_determinant_mu.  Returns the summed signs (`steps`);
+ * *value = steps * n_links / (4 n_attempts V), the printed number.
+ * Parity pinned by: field invariants (every link has its two ends, the
+ * sources close, n_links counts the links) and the ratios' full
+ * recomputes, test_worm_invariants. */
+long orc_worm(const orc_bag_lattice *B, int *field, orc_mt *st, double U, int n_attempts,
+              int *n_links, int *n_monomers, int *sign, double *value);
+
 #ifdef __cplusplus
 }
 #endif

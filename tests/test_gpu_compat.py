@@ -535,3 +535,58 @@ def test_bag_update_sequence(bag_lib, oracle):
     finally:
         C.c_void_p.in_dll(bag_lib, "field").value = None
         C.c_int.in_dll(bag_lib, "max_changes").value = 0
+
+
+@pytest.mark.parametrize("wmm,m", [(1, 0.0), (0, 0.3)])
+def test_worm_susceptibility_matches_oracle(bag_lib, oracle, wmm, m):
+    """thirring_bag_worm (measure_susceptibility, fermionbag.c:450-590) on
+    the program's field with the device ratios and hooks, against the
+    oracle's worm with full determinants: the same MT stream gives the same
+    summed signs, printed value, final field, n_links, n_monomers and
+    current_sign, measurement after measurement."""
+    nt = nx = 6
+    V = nt * nx
+    rng = np.random.default_rng(91 + wmm)
+    Bo = oracle.bag_lattice(nt, nx, with_mass_monomers=wmm, m=m)
+    while True:  # a configuration with det D_FF != 0 (m = 0 allows zeros)
+        f0 = np.zeros(V, np.int32)
+        for s in rng.permutation(V):
+            if f0[s] or rng.random() > 0.35:
+                continue
+            t, x = divmod(int(s), nx)
+            d = int(rng.integers(2))
+            s2 = ((t + 1) % nt) * nx + x if d == 0 else t * nx + (x + 1) % nx
+            if not f0[s2]:
+                f0[s], f0[s2] = 2 + d, 4 + d
+        if oracle.determinant_mu(Bo, f0)[1] != 0:
+            break
+    field, B = _bag_setup(bag_lib, oracle, nt, nx, 0, wmm, m, 0.0, f0)
+    U = 1.3
+    C.c_double.in_dll(bag_lib, "U").value = U
+    nl = C.c_int.in_dll(bag_lib, "n_links")
+    nm = C.c_int.in_dll(bag_lib, "n_monomers")
+    sign = C.c_int.in_dll(bag_lib, "current_sign")
+    nl.value = int(np.count_nonzero((f0 == 2) | (f0 == 3)))
+    nm.value = 0
+    bag_lib.thirring_bag_worm.restype = C.c_long
+    bag_lib.thirring_bag_worm.argtypes = [I, C.POINTER(D)]
+    bag_lib.seed_mersenne.argtypes = [C.c_long]
+    try:
+        bag_lib.update_background()
+        bag_lib.seed_mersenne(55)
+        r = oracle.Rng(55)
+        f, o_sign, o_nl, o_nm = f0.copy(), sign.value, nl.value, 0
+        _, s_ff, _ = oracle.determinant_mu(B, f0)
+        _, s_d, _ = oracle.determinant_mu(B, np.zeros(V, np.int32))
+        assert o_sign == s_ff * s_d
+        for _ in range(4):
+            v = D()
+            steps = bag_lib.thirring_bag_worm(5, C.byref(v))
+            o_steps, o_val, f, o_nl, o_nm, o_sign = oracle.worm(
+                B, f, r, U, n_attempts=5, n_links=o_nl, n_monomers=o_nm, sign=o_sign)
+            assert bag_lib.thirring_bag_last_status() == 0
+            assert steps == o_steps and abs(v.value - o_val) <= 1e-15 * max(1.0, abs(o_val))
+            assert np.array_equal(field.natural(), f)
+            assert (nl.value, nm.value, sign.value) == (o_nl, o_nm, o_sign)
+    finally:
+        C.c_void_p.in_dll(bag_lib, "field").value = None

@@ -417,3 +417,77 @@ def test_philox_uniform_moments(oracle):
         assert abs(u[:, k].var() - 1 / 12) < 5 * np.sqrt(1 / 180 / n)
         assert stats.kstest(u[:, k], "uniform").pvalue > 1e-4
     assert abs(np.corrcoef(u[:, 0], u[:, 1])[0, 1]) < 5 / np.sqrt(n)
+
+
+def _bag_config(nt, nx, rng, rho=0.35, monomer_pairs=2):
+    V = nt * nx
+    f = np.zeros(V, np.int32)
+    for s in rng.permutation(V):
+        if f[s] or rng.random() > rho:
+            continue
+        t, x = divmod(int(s), nx)
+        d = int(rng.integers(2))
+        s2 = ((t + 1) % nt) * nx + x if d == 0 else t * nx + (x + 1) % nx
+        if not f[s2]:
+            f[s], f[s2] = 2 + d, 4 + d
+    for _ in range(monomer_pairs):
+        free = [s for s in range(V) if f[s] == 0 and f[((s // nx + 1) % nt) * nx + s % nx] == 0]
+        if free:
+            s = free[int(rng.integers(len(free)))]
+            f[s] = f[((s // nx + 1) % nt) * nx + s % nx] = 1
+    return f
+
+
+def _legal(f, nt, nx):
+    """Every link value has its partner (tail 2+d at s, head 4+d at s+d)."""
+    V = nt * nx
+    for s in range(V):
+        t, x = divmod(s, nx)
+        v = f[s]
+        if v in (2, 3):
+            d = v - 2
+            s2 = ((t + 1) % nt) * nx + x if d == 0 else t * nx + (x + 1) % nx
+            if f[s2] != v + 2:
+                return False
+        elif v in (4, 5):
+            d = v - 4
+            s2 = ((t - 1) % nt) * nx + x if d == 0 else t * nx + (x - 1) % nx
+            if f[s2] != v - 2:
+                return False
+        elif v not in (0, 1):
+            return False
+    return True
+
+
+@pytest.mark.parametrize("wmm", [1, 0])
+def test_worm_invariants(oracle, wmm):
+    """orc_worm (fermionbag.c:450-590): after each measurement every link has
+    its two ends and no source monomer is left (every worm closes), the
+    monomers are conserved, n_links counts the links, the sign equals the
+    sign of det(D_FF) det(D) of the new configuration, and the occupied set
+    only changes through accepted link insertions and removals (their
+    weights U det ratios)."""
+    rng = np.random.default_rng(77)
+    nt = nx = 6
+    V = nt * nx
+    B = oracle.bag_lattice(nt, nx, with_mass_monomers=wmm, m=0.0 if wmm else 0.3)
+    la_d, sg_d, _ = oracle.determinant_mu(B, np.zeros(V, np.int32))
+    f = _bag_config(nt, nx, rng)
+    r = oracle.Rng(55)
+    _, s0, _ = oracle.determinant_mu(B, f)
+    sign = s0 * sg_d
+    nm0 = int(np.count_nonzero(f == 1))
+    changed = False
+    for _ in range(6):
+        steps, val, g, nl, nm, sign = oracle.worm(B, f, r, U=1.3, n_attempts=5,
+                                                  n_monomers=nm0, sign=sign)
+        assert _legal(g, nt, nx)
+        assert nl == np.count_nonzero((g == 2) | (g == 3)) and nm == nm0
+        assert np.count_nonzero(g == 1) == nm0
+        _, sg, _ = oracle.determinant_mu(B, g)
+        assert sign == sg * sg_d
+        assert steps != 0
+        changed |= not np.array_equal(g != 0, f != 0)
+        f = g
+    # occupations change only with mass monomers (new links / removals)
+    assert changed == bool(wmm)

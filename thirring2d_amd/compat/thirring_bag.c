@@ -31,6 +31,53 @@ __attribute__((weak)) int current_sign = 1;
 __attribute__((weak)) int max_changes = 0;
 __attribute__((weak)) void update_linklists(void) {}
 
+/* ---- MT19937 stream of mersenne.h (weak: a program's mersenne_inline.c,
+ * or libthirring_hmc's copy when both libraries are loaded, takes over) --
+ * Matsumoto-Nishimura MT19937 with the Knuth LCG (69069) seeding of the
+ * 1998 sgenrand; blocks of 624 doubles y / 2^32 consumed from the top. */
+#define MT_N 624
+#define MT_M 397
+__attribute__((weak)) int mersenne_i = -1;
+__attribute__((weak)) double mersenne_array[MT_N];
+static unsigned int th_mt[MT_N];
+
+static void th_seed(long seed)
+{
+  if (seed % 2 == 0) seed++;
+  th_mt[0] = (unsigned int)(seed & 0xffffffffL);
+  for (int k = 1; k < MT_N; k++) th_mt[k] = 69069u * th_mt[k - 1];
+  mersenne_i = 0;
+}
+
+static double th_generate(void)
+{
+  if (mersenne_i < 0) th_seed(4357);
+  for (int k = 0; k < MT_N; k++) {
+    unsigned int y = (th_mt[k] & 0x80000000u) | (th_mt[(k + 1) % MT_N] & 0x7fffffffu);
+    unsigned int v = th_mt[(k + MT_M) % MT_N] ^ (y >> 1);
+    if (y & 1u) v ^= 0x9908b0dfu;
+    th_mt[k] = v;
+  }
+  for (int k = 0; k < MT_N; k++) {
+    unsigned int y = th_mt[k];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    mersenne_array[k] = (double)y * 2.3283064365386963e-10;
+  }
+  mersenne_i = MT_N;
+  return mersenne_array[--mersenne_i];
+}
+
+void seed_mersenne(long seed) __attribute__((weak, alias("th_seed")));
+double mersenne_generate(void) __attribute__((weak, alias("th_generate")));
+
+static inline double draw(void)
+{
+  return mersenne_i > 0 ? mersenne_array[--mersenne_i] : mersenne_generate();
+}
+
 static struct {
   th2d_ctx *ctx;
   th2d_bag_geom geo;
@@ -429,4 +476,161 @@ void not_moved_source_monomer(int t2, int x2, int t4, int x4)
   (void)x2;
   (void)t4;
   (void)x4;
+}
+
+/* ---- fermionbag.c: the worm estimate of the susceptibility (:450-590) ----
+ * n_attempts worms on the program's field: a random link (in the order
+ * update_linklists lists them) becomes two source monomers, one of which
+ * moves until it meets the other and the link closes again.  The moves,
+ * the draws and their order are the reference's; the determinant ratios
+ * (det_added_link, det_removed_link, det_moved_monomer) and the accept
+ * hooks (new_link, removed_link) are this library's.  Antisymmetric
+ * lattices only (the open-x neighbour arrays point at a column that does
+ * not exist). */
+#define LINK_TUP 2
+
+static int worm_links(int *links)
+{
+  int n = 0;
+  for (int t = 0; t < S.nt; t++)
+    for (int x = 0; x < S.nx; x++)
+      if (field[t][x] >= LINK_TUP && field[t][x] < LINK_TUP + 2)
+        links[n++] = (field[t][x] - LINK_TUP) * S.V + t * S.nx + x;
+  return n;
+}
+
+static void worm_link_on(int t, int x, int dir)
+{
+  int t2, x2;
+  nb(t, x, dir, &t2, &x2);
+  field[t][x] = dir + 2;
+  field[t2][x2] = (dir + 2) % 4 + 2;
+}
+
+static void worm_link_off(int t, int x, int dir)
+{
+  int t2, x2;
+  nb(t, x, dir, &t2, &x2);
+  field[t][x] = 0;
+  field[t2][x2] = 0;
+}
+
+long thirring_bag_worm(int n_attempts, double *value)
+{
+  if (value) *value = 0.0;
+  if (n_attempts < 1 || !field || !state() || S.geo.bc != TH2D_BC_ANTISYMMETRIC) {
+    S.status = TH2D_EARG;
+    return 0;
+  }
+  int *links = malloc(sizeof(int) * 2 * (size_t)S.V);
+  if (!links) {
+    S.status = TH2D_ENOMEM;
+    return 0;
+  }
+  long steps = 0;
+  const double scale = (double)n_links / (4.0 * n_attempts * S.V);
+  update_linklists();
+  n_links = worm_links(links);
+  if (n_links > 0)
+    for (int attempt = 0; attempt < n_attempts && S.status == TH2D_OK; attempt++) {
+      const int l = links[(int)(draw() * n_links)];
+      int x1 = l % S.nx, t1 = (l / S.nx) % S.nt, nu = l / S.V, t2, x2;
+      if (draw() > 0.5) {
+        int ta, xa;
+        nb(t1, x1, nu, &ta, &xa);
+        t1 = ta;
+        x1 = xa;
+      }
+      const int dir0 = field[t1][x1] - LINK_TUP;
+      worm_link_off(t1, x1, dir0);
+      nb(t1, x1, dir0, &t2, &x2);
+      field[t1][x1] = SOURCE_MONOMER;
+      field[t2][x2] = SOURCE_MONOMER;
+      n_monomers += 2;
+      n_links -= 1;
+      steps += current_sign;
+      for (;;) {
+        const int dir = (int)(4 * draw());
+        int t3, x3;
+        nb(t2, x2, dir, &t3, &x3);
+        if (draw() > 0.5) {
+          if (t3 == t1 && x3 == x1) {  /* back at the start: the link again */
+            field[t1][x1] = 0;
+            field[t2][x2] = 0;
+            worm_link_on(t2, x2, dir);
+            n_monomers -= 2;
+            n_links += 1;
+            break;
+          } else if (field[t3][x3] >= LINK_TUP) {  /* flip the link */
+            const int ld = field[t3][x3] - LINK_TUP;
+            int t4, x4;
+            nb(t3, x3, ld, &t4, &x4);
+            field[t2][x2] = 0;
+            worm_link_off(t3, x3, ld);
+            worm_link_on(t2, x2, dir);
+            field[t4][x4] = SOURCE_MONOMER;
+            t2 = t4;
+            x2 = x4;
+          } else if (field[t3][x3] == 1) {  /* swap with the monomer */
+            field[t2][x2] = 1;
+            field[t3][x3] = SOURCE_MONOMER;
+            t2 = t3;
+            x2 = x3;
+          }
+        } else if (S.geo.with_mass_monomers) {
+          int dir2;
+          do dir2 = (int)(4 * draw());
+          while (dir2 == (dir + 2) % 4);
+          int t4, x4;
+          nb(t3, x3, dir2, &t4, &x4);
+          (void)draw(); /* the hop of :532-544 is never taken (draw < 0) */
+          if (field[t3][x3] == 0 && field[t4][x4] == 0) {
+            const double p = U * det_added_link(t3, x3, t4, x4);
+            if (draw() < p) {
+              new_link(t3, x3, dir2);
+              field[t2][x2] = 0;
+              worm_link_off(t3, x3, dir2);
+              worm_link_on(t2, x2, dir);
+              field[t4][x4] = SOURCE_MONOMER;
+              t2 = t4;
+              x2 = x4;
+            }
+          } else if (field[t3][x3] == LINK_TUP + dir2) {
+            const double p = det_removed_link(t2, x2, t3, x3) / U;
+            if (draw() < p) {
+              field[t2][x2] = 0;
+              worm_link_off(t3, x3, dir2);
+              worm_link_on(t2, x2, dir);
+              field[t4][x4] = SOURCE_MONOMER;
+              removed_link(t2, x2, dir);
+              t2 = t4;
+              x2 = x4;
+            }
+          }
+        } else if (field[t3][x3] == 0) {
+          /* the hop to an empty neighbour (:565-577) */
+          const double det = fabs(det_moved_monomer(t2, x2, t3, x3));
+          if (draw() < det) {
+            moved_source_monomer(t2, x2, t3, x3);
+            t2 = t3;
+            x2 = x3;
+          } else {
+            not_moved_source_monomer(t2, x2, t3, x3);
+          }
+        }
+        steps += current_sign;
+      }
+      update_linklists();
+      n_links = worm_links(links);
+    }
+  free(links);
+  if (value) *value = (double)steps * scale;
+  return steps;
+}
+
+__attribute__((weak)) void measure_susceptibility(void)
+{
+  double v = 0.0;
+  thirring_bag_worm(20, &v);
+  printf("Susceptibility %g \n", v);
 }
