@@ -177,7 +177,7 @@ int th2d_cg_last_tile(th2d_ctx *ctx, int *tt, int *tx);
  * cost model prices it faster, or when a tile is set with
  * th2d_set_cg_tile.  th2d_cg_last_resident tells which kernel the last
  * inner solve used: 0 the step kernel, 1 the resident kernel, 2 both (a
- * split batch). */
+ * split batch), 3 the small-lattice kernel (FP64, th2d_set_cg_small). */
 int th2d_set_cg_resident(th2d_ctx *ctx, int on);
 /* Step-kernel launches as captured graphs (0 off, default; 1 on): the
  * step kernel's iterations are issued as replays of a hipGraph holding 8
@@ -201,6 +201,13 @@ int th2d_set_cg_graph(th2d_ctx *ctx, int on);
  * by the end of a pass; hence off by default. */
 int th2d_set_cg_reliable(th2d_ctx *ctx, int on);
 int th2d_cg_last_resident(th2d_ctx *ctx, int *on);
+/* Small lattices (1, default; 0 off): an FP64 inner solve of a batch whose
+ * lattices have at most 2048 sites runs as one launch, one workgroup per
+ * lattice, the whole CG loop inside it (vectors in registers, the operator
+ * through LDS); th2d_cg_last_resident then reports 3.  Off, or with a tile
+ * set by th2d_set_cg_tile, the step kernel solves them.  Replaces no
+ * reference function. */
+int th2d_set_cg_small(th2d_ctx *ctx, int on);
 /* Bounded waits of the resident solver.  Every wait between the regions
  * of a chain gives up after `polls` polls (polls < 0: the default, 2^20,
  * about 1 s) and ends every workgroup of the launch normally; the solve is

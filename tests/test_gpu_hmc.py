@@ -384,6 +384,38 @@ def test_cg_resident_full_device(hmc, oracle):
         assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
 
 
+def test_cg_small_lattice_kernel(hmc, oracle):
+    """FP64 solves of lattices up to 2048 sites run the whole CG in one
+    workgroup (th2d_set_cg_small): a batch of uneven small lattices (1 to 2
+    sites per thread, 256 and 1024 threads) reaches the true residual, the
+    solutions agree with the oracle's, and the step kernel (small path off)
+    solves the same batch to the same tolerance with iteration counts within
+    2%."""
+    from thirring2d_amd import _native as N
+    dims = [(16, 16), (8, 8), (24, 40), (32, 64), (6, 10), (32, 16)]
+    setups = [_setup(hmc, oracle, nt, nx, 300 + k, m=0.1 + 0.03 * k)
+              for k, (nt, nx) in enumerate(dims)]
+    batch = hmc.ChainBatch([s[0] for s in setups])
+    ctx = setups[0][0].ctx
+    runs = {}
+    try:
+        for on in (1, 0):
+            N.check(N.lib().th2d_set_cg_small(ctx.ptr, on), "small")
+            xs, infos = batch.fmdm_invert_cg([s[3] for s in setups],
+                                             hmc.cg_params(tol_rel=1e-12))
+            assert _last_resident(ctx) == (3 if on else 0)
+            runs[on] = (xs, infos)
+    finally:
+        N.lib().th2d_set_cg_small(ctx.ptr, 1)
+    for k, (lat, L, A, b, _) in enumerate(setups):
+        inf = runs[1][1][k]
+        assert inf.status == 0 and inf.true_rel_resid <= 1e-12
+        xo, _, _ = oracle.cg_mdm(L, A, b, tol_rel=1e-13)
+        assert _rel(runs[1][0][k], xo) <= 1e-9
+        it1, it0 = inf.iterations, runs[0][1][k].iterations
+        assert abs(it1 - it0) <= max(2, it0 // 50)
+
+
 def test_cg_resident_wait_timeout_falls_back(hmc, oracle):
     """A bounded wait of the resident solver that gives up (forced from the
     host with a spin limit of 0 polls: every workgroup still exits normally)

@@ -1153,6 +1153,479 @@ struct Carry {
   double cscale, gam0, alpha0;
 };
 
+// ---- small lattices: a whole FP64 inner solve in one workgroup ----------
+// One workgroup of NT threads per chain, SPT sites per thread (V <= NT SPT,
+// at most 2048 sites):
+// x, r, p and the site's four folded links stay in registers; the operator
+// reads its neighbours from one LDS copy of the vector it is applied to;
+// plain CG (two block sums per iteration) runs until r.r < stop or
+// max_iter, in one launch for every chain of the batch.  For the small
+// lattices of the reference's own runs (16^2: one site per thread) this
+// replaces a launch and a host check per iteration.
+struct SmallChain {
+  Geo G;
+  const double2 *b, *K;
+  double2 *x;
+  double stop;
+  int max_iter, pad;
+};
+
+template <int NW>
+__device__ __forceinline__ double small_sum(double v, double *red)
+{
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < NW; w++) s += red[w];  // every thread, same order
+  return s;
+}
+
+template <int NT, int SPT, bool KREG>
+__global__ void __launch_bounds__(NT) k_cg_small(const SmallChain *chains, CgResult *out)
+{
+  constexpr int NW = NT / 64;
+  __shared__ double2 buf[NT * SPT];
+  __shared__ double red[2][NW];
+  const SmallChain C = chains[blockIdx.x];
+  const Geo G = C.G;
+  const int V = 2 * G.vh;
+  double2 x[SPT], r[SPT], p[SPT], q[SPT], kt[SPT], kx[SPT], ktd[SPT], kxd[SPT];
+  int itu[SPT], itd[SPT], ixu[SPT], ixd[SPT];
+  double rr = 0.0;
+#pragma unroll
+  for (int k = 0; k < SPT; k++) {
+    const int i = threadIdx.x + k * NT;
+    x[k] = r[k] = p[k] = q[k] = make_double2(0.0, 0.0);
+    kt[k] = kx[k] = ktd[k] = kxd[k] = make_double2(0.0, 0.0);
+    itu[k] = itd[k] = ixu[k] = ixd[k] = 0;
+    if (i < V) {
+      const EoSite st = eo_site(G, i);
+      itu[k] = st.itu;
+      itd[k] = st.itd;
+      ixu[k] = st.ixu;
+      ixd[k] = st.ixd;
+      if (KREG) {  // else (1024 threads, 4 sites each) read in every apply
+        kt[k] = C.K[i];
+        kx[k] = C.K[V + i];
+        ktd[k] = C.K[st.itd];
+        kxd[k] = C.K[V + st.ixd];
+      }
+      r[k] = p[k] = C.b[i];
+      rr += r[k].x * r[k].x + r[k].y * r[k].y;
+    }
+  }
+  rr = small_sum<NW>(rr, red[0]);
+  int it = 0, status = TH2D_OK;
+  auto apply = [&](int dagger, double2 *dst) {
+    const double fu = dagger ? -G.em : G.ep, fd = dagger ? -G.ep : G.em, sx = dagger ? -1.0 : 1.0;
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+      const int i = threadIdx.x + k * NT;
+      if (i >= V) continue;
+      const double2 a = KREG ? kt[k] : C.K[i], b = KREG ? ktd[k] : C.K[itd[k]];
+      const double2 c = KREG ? kx[k] : C.K[V + i], d = KREG ? kxd[k] : C.K[V + ixd[k]];
+      double2 v = cscale(buf[i], G.m);
+      v = cadd(v, cscale(cmul(a, buf[itu[k]]), fu));
+      v = csub(v, cscale(cjmul(b, buf[itd[k]]), fd));
+      v = cadd(v, cscale(cmul(c, buf[ixu[k]]), sx));
+      v = csub(v, cscale(cjmul(d, buf[ixd[k]]), sx));
+      dst[k] = v;
+    }
+  };
+  while (!(rr < C.stop) && it < C.max_iter) {
+    double2 y[SPT];
+#pragma unroll
+    for (int k = 0; k < SPT; k++)
+      if (threadIdx.x + k * NT < V) buf[threadIdx.x + k * NT] = p[k];
+    __syncthreads();
+    apply(0, y);  // y = M p
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SPT; k++)
+      if (threadIdx.x + k * NT < V) buf[threadIdx.x + k * NT] = y[k];
+    __syncthreads();
+    apply(1, q);  // q = M^dag y
+    double pq = 0.0;
+#pragma unroll
+    for (int k = 0; k < SPT; k++)
+      if (threadIdx.x + k * NT < V) pq += p[k].x * q[k].x + p[k].y * q[k].y;
+    pq = small_sum<NW>(pq, red[0]);
+    if (!(pq > 0.0) || !isfinite(pq)) {
+      status = TH2D_EBREAKDOWN;
+      break;
+    }
+    const double alpha = rr / pq;
+    double rn = 0.0;
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+      x[k] = cadd(x[k], cscale(p[k], alpha));
+      r[k] = csub(r[k], cscale(q[k], alpha));
+      rn += r[k].x * r[k].x + r[k].y * r[k].y;
+    }
+    rn = small_sum<NW>(rn, red[1]);
+    it++;
+    const double beta = rn / rr;
+    rr = rn;
+    if (rr < C.stop) break;
+#pragma unroll
+    for (int k = 0; k < SPT; k++) p[k] = cadd(r[k], cscale(p[k], beta));
+  }
+  if (status == TH2D_OK && !(rr < C.stop)) status = TH2D_ENOCONV;
+#pragma unroll
+  for (int k = 0; k < SPT; k++)
+    if (threadIdx.x + k * NT < V) C.x[threadIdx.x + k * NT] = x[k];
+  if (threadIdx.x == 0) {
+    CgResult o;
+    memset(&o, 0, sizeof(o));
+    o.iters = it;
+    o.status = status;
+    o.rr = rr;
+    o.done = 1;
+    out[blockIdx.x] = o;
+  }
+}
+
+// The whole FP64 solve of th2d_cg_mdagm_batched on a small lattice in one
+// workgroup: |b|^2, the stop target, CG on (M^dag M) x = b from x = 0 in
+// the Chronopoulos-Gear form (both inner products of an iteration in one
+// block sum), then the true residual b - M^dag M x in FP64 and, while it
+// misses the target, restarts from it, as the host loop does for larger
+// lattices.  x is written to the caller's vector.
+struct SmallFull {
+  Geo G;
+  const double2 *b, *K;
+  double2 *x;
+  double tol_rel, tol_abs_rr;
+  int max_iter, pad;
+};
+
+struct SmallInfo {
+  int iters, restarts, status, pad;
+  double true_rel, rr;
+};
+
+template <int NT, int SPT>
+__global__ void __launch_bounds__(NT) k_cg_small_full(const SmallFull *chains, SmallInfo *out)
+{
+  constexpr int NW = NT / 64;
+  __shared__ double2 buf[NT * SPT];
+  __shared__ double red[2][2 * NW];
+  const SmallFull C = chains[blockIdx.x];
+  const Geo G = C.G;
+  const int V = 2 * G.vh;
+  double2 x[SPT], r[SPT], p[SPT], s[SPT], kt[SPT], kx[SPT], ktd[SPT], kxd[SPT];
+  int itu[SPT], itd[SPT], ixu[SPT], ixd[SPT];
+  double bb = 0.0;
+#pragma unroll
+  for (int k = 0; k < SPT; k++) {
+    const int i = threadIdx.x + k * NT;
+    x[k] = r[k] = p[k] = s[k] = make_double2(0.0, 0.0);
+    kt[k] = kx[k] = ktd[k] = kxd[k] = make_double2(0.0, 0.0);
+    itu[k] = itd[k] = ixu[k] = ixd[k] = 0;
+    if (i < V) {
+      const EoSite st = eo_site(G, i);
+      itu[k] = st.itu;
+      itd[k] = st.itd;
+      ixu[k] = st.ixu;
+      ixd[k] = st.ixd;
+      kt[k] = C.K[i];
+      kx[k] = C.K[V + i];
+      ktd[k] = C.K[st.itd];
+      kxd[k] = C.K[V + st.ixd];
+      r[k] = C.b[i];
+      bb += r[k].x * r[k].x + r[k].y * r[k].y;
+    }
+  }
+  int rb = 0;  // which reduction buffer; alternates, a barrier between reuses
+  auto sum2 = [&](double a, double b2) {
+    a = wave_sum(a);
+    b2 = wave_sum(b2);
+    double *rd = red[rb];
+    rb ^= 1;
+    if ((threadIdx.x & 63) == 0) {
+      rd[2 * (threadIdx.x >> 6)] = a;
+      rd[2 * (threadIdx.x >> 6) + 1] = b2;
+    }
+    __syncthreads();
+    double2 o = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+      o.x += rd[2 * w];
+      o.y += rd[2 * w + 1];
+    }
+    return o;
+  };
+  // w = M^dag M v for the thread's sites, v given per site
+  auto mdagm = [&](const double2 *v, double2 *w) {
+    double2 y[SPT];
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++) {
+      const double2 *src = pass == 0 ? v : y;
+#pragma unroll
+      for (int k = 0; k < SPT; k++)
+        if (threadIdx.x + k * NT < V) buf[threadIdx.x + k * NT] = src[k];
+      __syncthreads();
+      const double fu = pass ? -G.em : G.ep, fd = pass ? -G.ep : G.em, sx = pass ? -1.0 : 1.0;
+      double2 o[SPT];
+#pragma unroll
+      for (int k = 0; k < SPT; k++) {
+        const int i = threadIdx.x + k * NT;
+        o[k] = make_double2(0.0, 0.0);
+        if (i >= V) continue;
+        double2 t = cscale(buf[i], G.m);
+        t = cadd(t, cscale(cmul(kt[k], buf[itu[k]]), fu));
+        t = csub(t, cscale(cjmul(ktd[k], buf[itd[k]]), fd));
+        t = cadd(t, cscale(cmul(kx[k], buf[ixu[k]]), sx));
+        t = csub(t, cscale(cjmul(kxd[k], buf[ixd[k]]), sx));
+        o[k] = t;
+      }
+      __syncthreads();  // every thread has read buf before it is rewritten
+#pragma unroll
+      for (int k = 0; k < SPT; k++) (pass == 0 ? y : w)[k] = o[k];
+    }
+  };
+  bb = sum2(bb, 0.0).x;
+  const bool abs_mode = C.tol_rel <= 0;
+  double target = 0.0;
+  if (C.tol_rel > 0) target = C.tol_rel * C.tol_rel * bb;
+  if (C.tol_abs_rr > 0) target = fmax(target, C.tol_abs_rr);
+  int it = 0, restarts = 0, status = TH2D_OK;
+  double rtrue = bb, rr = bb;
+  if (bb > 0.0 && !(bb <= target)) {
+    for (int pass = 0; pass < 64; pass++) {
+      // CG from x with r the current residual: p = s = 0, then per
+      // iteration w = A r, gamma = r.r, delta = r.w (one block sum)
+      double gam_old = 1.0, alpha_old = 1.0;
+      bool broke = false;
+#pragma unroll
+      for (int k = 0; k < SPT; k++) p[k] = s[k] = make_double2(0.0, 0.0);
+      for (int j = 0; it < C.max_iter; j++) {
+        double2 w[SPT];
+        mdagm(r, w);
+        double g = 0.0, d = 0.0;
+#pragma unroll
+        for (int k = 0; k < SPT; k++) {
+          g += r[k].x * r[k].x + r[k].y * r[k].y;
+          d += r[k].x * w[k].x + r[k].y * w[k].y;
+        }
+        const double2 gd = sum2(g, d);
+        rr = gd.x;
+        if (rr < target) break;  // the recursive residual meets the target
+        const double beta = j == 0 ? 0.0 : gd.x / gam_old;
+        const double den = gd.y - beta * gd.x / alpha_old;
+        if (!(den > 0.0) || !isfinite(den)) {
+          broke = true;
+          break;
+        }
+        const double alpha = gd.x / den;
+#pragma unroll
+        for (int k = 0; k < SPT; k++) {
+          p[k] = cadd(r[k], cscale(p[k], beta));
+          s[k] = cadd(w[k], cscale(s[k], beta));
+          x[k] = cadd(x[k], cscale(p[k], alpha));
+          r[k] = csub(r[k], cscale(s[k], alpha));
+        }
+        gam_old = gd.x;
+        alpha_old = alpha;
+        it++;
+      }
+      // the true residual in FP64
+      double2 ax[SPT];
+      mdagm(x, ax);
+      double rn = 0.0;
+#pragma unroll
+      for (int k = 0; k < SPT; k++) {
+        const int i = threadIdx.x + k * NT;
+        r[k] = i < V ? csub(C.b[i], ax[k]) : make_double2(0.0, 0.0);
+        rn += r[k].x * r[k].x + r[k].y * r[k].y;
+      }
+      rn = sum2(rn, 0.0).x;
+      if (pass > 0) restarts++;
+      if (abs_mode && !broke) {  // the reference's stop on the recursive r.r
+        rtrue = rn;
+        break;
+      }
+      if (broke && !(rn < rtrue)) {
+        status = TH2D_EBREAKDOWN;
+        rtrue = rn;
+        break;
+      }
+      if (!(rn < rtrue) && pass > 2) {  // stagnation at round-off
+        rtrue = rn;
+        break;
+      }
+      rtrue = rn;
+      if (rtrue <= target) break;
+      if (it >= C.max_iter) {
+        status = TH2D_ENOCONV;
+        break;
+      }
+    }
+  }
+  if (status == TH2D_OK && rtrue > target && !abs_mode && bb > 0) status = TH2D_ENOCONV;
+#pragma unroll
+  for (int k = 0; k < SPT; k++)
+    if (threadIdx.x + k * NT < V) C.x[threadIdx.x + k * NT] = x[k];
+  if (threadIdx.x == 0) {
+    SmallInfo o;
+    o.iters = it;
+    o.restarts = restarts;
+    o.status = status;
+    o.pad = 0;
+    o.true_rel = bb > 0 ? sqrt(rtrue / bb) : 0.0;
+    o.rr = rr;
+    out[blockIdx.x] = o;
+  }
+}
+
+constexpr long kSmallMaxSites = 2048;  // 1024 threads x 2 sites without spills
+
+bool small_fits(int nch, const Geo *G)
+{
+  for (int c = 0; c < nch; c++)
+    if (2L * G[c].vh > kSmallMaxSites) return false;
+  return nch > 0;
+}
+
+// The batched inner solve (A_c y_c = rhs_c from y_c = 0, stop r.r < stop_c)
+// on small lattices, one workgroup per chain; solutions in xout[c].
+int small_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
+                const double2 *const *rhs, const double2 *const *K, const double *stop_rr,
+                const int *max_iter, double2 **xout, InnerResult *res)
+{
+  long maxV = 0, voff = 0;
+  std::vector<long> off(nch);
+  for (int c = 0; c < nch; c++) {
+    const long V = 2L * G[c].vh;
+    maxV = std::max(maxV, V);
+    off[c] = voff;
+    voff += (V + 63) & ~63L;
+  }
+  auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+  const size_t vb = al(sizeof(double2) * (size_t)voff), cb = al(sizeof(SmallChain) * nch);
+  TH2D_TRY(th2d_buf_reserve(buf, vb + cb + sizeof(CgResult) * nch));
+  double2 *X = (double2 *)buf.ptr;
+  SmallChain *dch = (SmallChain *)((char *)buf.ptr + vb);
+  CgResult *dres = (CgResult *)((char *)buf.ptr + vb + cb);
+  std::vector<SmallChain> ch(nch);
+  for (int c = 0; c < nch; c++) {
+    memset(&ch[c], 0, sizeof(SmallChain));
+    ch[c].G = G[c];
+    ch[c].b = rhs[c];
+    ch[c].K = K[c];
+    ch[c].x = X + off[c];
+    ch[c].stop = stop_rr[c];
+    ch[c].max_iter = max_iter[c];
+    xout[c] = X + off[c];
+  }
+  TH2D_HIP(hipMemcpyAsync(dch, ch.data(), sizeof(SmallChain) * nch, hipMemcpyHostToDevice,
+                          ctx->stream));
+  if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k0, ctx->stream));
+  if (maxV <= 256)
+    hipLaunchKernelGGL((k_cg_small<256, 1, true>), dim3(nch), dim3(256), 0, ctx->stream, dch, dres);
+  else if (maxV <= 512)
+    hipLaunchKernelGGL((k_cg_small<256, 2, true>), dim3(nch), dim3(256), 0, ctx->stream, dch, dres);
+  else if (maxV <= 1024)
+    hipLaunchKernelGGL((k_cg_small<256, 4, true>), dim3(nch), dim3(256), 0, ctx->stream, dch, dres);
+  else
+    hipLaunchKernelGGL((k_cg_small<1024, 2, true>), dim3(nch), dim3(1024), 0, ctx->stream, dch, dres);
+  TH2D_HIP(hipGetLastError());
+  const size_t rb = sizeof(CgResult) * nch;
+  TH2D_TRY(th2d_host_reserve(ctx->redh, rb + 64));
+  TH2D_HIP(hipMemcpyAsync(ctx->redh.ptr, dres, rb, hipMemcpyDeviceToHost, ctx->stream));
+  if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k1, ctx->stream));
+  TH2D_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<CgResult> h(nch);
+  memcpy(h.data(), ctx->redh.ptr, rb);
+  if (ctx->ktiming) {
+    float ms = 0.f;
+    TH2D_HIP(hipEventElapsedTime(&ms, ctx->k0, ctx->k1));
+    ctx->ktime_ms += ms;
+    ctx->klaunches += 1;
+    for (int c = 0; c < nch; c++) {
+      ctx->kbytes += (double)h[c].iters * 2.0 * G[c].vh * 240.0;
+      ctx->kiters += h[c].iters;
+    }
+  }
+  for (int c = 0; c < nch; c++) {
+    memset(&res[c], 0, sizeof(InnerResult));
+    res[c].iters = h[c].iters;
+    res[c].rr = h[c].rr;
+    res[c].status = h[c].status;
+  }
+  ctx->cg_last_resident = 3;
+  return TH2D_OK;
+}
+
+// th2d_cg_mdagm_batched in FP64 for small lattices: one launch.
+int small_full_solve(th2d_ctx *ctx, int nch, const Geo *G, double **x, const double *const *b,
+                     const double *const *K, const th2d_cg_params *p, th2d_cg_info *infos)
+{
+  long maxV = 0;
+  for (int c = 0; c < nch; c++) maxV = std::max(maxV, 2L * G[c].vh);
+  auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+  const size_t cb = al(sizeof(SmallFull) * nch);
+  TH2D_TRY(th2d_buf_reserve(ctx->cg64, cb + sizeof(SmallInfo) * nch));
+  SmallFull *dch = (SmallFull *)ctx->cg64.ptr;
+  SmallInfo *dinf = (SmallInfo *)((char *)ctx->cg64.ptr + cb);
+  std::vector<SmallFull> ch(nch);
+  for (int c = 0; c < nch; c++) {
+    memset(&ch[c], 0, sizeof(SmallFull));
+    ch[c].G = G[c];
+    ch[c].b = (const double2 *)b[c];
+    ch[c].K = (const double2 *)K[c];
+    ch[c].x = (double2 *)x[c];
+    ch[c].tol_rel = p->tol_rel;
+    ch[c].tol_abs_rr = p->tol_abs_rr;
+    ch[c].max_iter = p->max_iter;
+  }
+  TH2D_HIP(hipMemcpyAsync(dch, ch.data(), sizeof(SmallFull) * nch, hipMemcpyHostToDevice,
+                          ctx->stream));
+  if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k0, ctx->stream));
+  if (maxV <= 256)
+    hipLaunchKernelGGL((k_cg_small_full<256, 1>), dim3(nch), dim3(256), 0, ctx->stream, dch, dinf);
+  else if (maxV <= 512)
+    hipLaunchKernelGGL((k_cg_small_full<256, 2>), dim3(nch), dim3(256), 0, ctx->stream, dch, dinf);
+  else if (maxV <= 1024)
+    hipLaunchKernelGGL((k_cg_small_full<256, 4>), dim3(nch), dim3(256), 0, ctx->stream, dch, dinf);
+  else
+    hipLaunchKernelGGL((k_cg_small_full<1024, 2>), dim3(nch), dim3(1024), 0, ctx->stream, dch,
+                       dinf);
+  TH2D_HIP(hipGetLastError());
+  if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k1, ctx->stream));
+  const size_t rb = sizeof(SmallInfo) * nch;
+  TH2D_TRY(th2d_host_reserve(ctx->redh, rb + 64));
+  TH2D_HIP(hipMemcpyAsync(ctx->redh.ptr, dinf, rb, hipMemcpyDeviceToHost, ctx->stream));
+  TH2D_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<SmallInfo> h(nch);
+  memcpy(h.data(), ctx->redh.ptr, rb);
+  if (ctx->ktiming) {
+    float ms = 0.f;
+    TH2D_HIP(hipEventElapsedTime(&ms, ctx->k0, ctx->k1));
+    ctx->ktime_ms += ms;
+    ctx->klaunches += 1;
+    for (int c = 0; c < nch; c++) {
+      ctx->kbytes += (double)h[c].iters * 2.0 * G[c].vh * 240.0;
+      ctx->kiters += h[c].iters;
+    }
+  }
+  ctx->cg_last_resident = 3;
+  int worst = TH2D_OK;
+  for (int c = 0; c < nch; c++) {
+    if (infos) {
+      infos[c].iterations = h[c].iters;
+      infos[c].restarts = h[c].restarts;
+      infos[c].status = h[c].status;
+      infos[c].true_rel_resid = h[c].true_rel;
+      infos[c].rr = h[c].rr;
+    }
+    if (h[c].status != TH2D_OK && worst == TH2D_OK) worst = h[c].status;
+  }
+  return worst;
+}
+
 // Batched inner solve: A_c y_c = rhs_c from y_c = 0, stop r.r < stop_c.
 // The solutions are the internal x vectors, returned in xout[c].
 // force_shape: -1 automatic (resident kernel where it applies), -2 step
@@ -1764,6 +2237,13 @@ extern "C" int th2d_cg_last_resident(th2d_ctx *ctx, int *on)
   return TH2D_OK;
 }
 
+extern "C" int th2d_set_cg_small(th2d_ctx *ctx, int on)
+{
+  if (!ctx) return TH2D_EARG;
+  ctx->cg_small = on ? 1 : 0;
+  return TH2D_OK;
+}
+
 extern "C" int th2d_set_cg_spin_limit(th2d_ctx *ctx, long polls)
 {
   if (!ctx || polls > (long)0xFFFFFFFFu) return TH2D_EARG;
@@ -1821,6 +2301,8 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
     voff[c + 1] = voff[c] + ((V[c] + 63) & ~63L);
   }
   const long Vt = voff[nch];
+  if (!mixed && ctx->cg_small && small_fits(nch, G.data()) && !(ctx->tile_t > 0 && ctx->tile_x > 0))
+    return small_full_solve(ctx, nch, G.data(), x, b, K, p, infos);
   // FP64 work: residuals r (all chains), tmp, per-chain scalars
   TH2D_TRY(th2d_buf_reserve(ctx->misc, sizeof(double) * 2 * 2 * Vt + sizeof(double) * (nch + 64)));
   double *R = (double *)ctx->misc.ptr;
@@ -1963,8 +2445,12 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
         rh[k] = (const double2 *)(outer == 0 ? b[c] : TMP + 2 * voff[c]);
         kk[k] = (const double2 *)K[c];
       }
-      TH2D_TRY(inner_solve<double>(ctx, ctx->cg64, outer, na, g2.data(), rh.data(), kk.data(),
-                                   stop.data(), mi.data(), xo.data(), ir.data()));
+      if (ctx->cg_small && small_fits(na, g2.data()) && !(ctx->tile_t > 0 && ctx->tile_x > 0))
+        TH2D_TRY(small_solve(ctx, ctx->cg64, na, g2.data(), rh.data(), kk.data(), stop.data(),
+                             mi.data(), xo.data(), ir.data()));
+      else
+        TH2D_TRY(inner_solve<double>(ctx, ctx->cg64, outer, na, g2.data(), rh.data(), kk.data(),
+                                     stop.data(), mi.data(), xo.data(), ir.data()));
       for (int k = 0; k < na; k++) {
         tx[k] = op_of(idx[k]);
         tx[k].d0 = x[idx[k]];
