@@ -113,6 +113,13 @@ int th2d_set_dense_mfma(int on);
  * interchanges (LAPACK getrf); the switch is there to compare them.
  * Replaces no reference function. */
 int th2d_set_lu_panel(int reg);
+/* Look-ahead of the blocked register-panel LU (process-wide; 1 default,
+ * 0 off): a block's panels run on a stream confined to 64 CUs (8 per XCD)
+ * while the rest of the previous block's trailing update runs on the other
+ * CUs; the block's row interchanges outside its own columns wait until that
+ * update is done.  The factors and pivots are the same either way.
+ * Replaces no reference function. */
+int th2d_set_lu_lookahead(int on);
 int th2d_copy_f64_to_f32(th2d_ctx *ctx, float *dst, const double *src, long n);
 
 /* ---- matrices of bag configurations ---------------------------------- */

@@ -174,6 +174,29 @@ def test_lu_factors_match_lapack(bag, native, reg):
         o += n
 
 
+def test_lu_lookahead_bitwise(bag, native):
+    """The look-ahead LU (panels on their own CUs beside the previous
+    block's trailing update, interchanges outside a block deferred) gives
+    bitwise the factors and pivots of the in-order LU, FP64 and FP32, on
+    uneven orders across block edges."""
+    rng = np.random.default_rng(21)
+    mats = _mats(rng, [2304, 129, 300, 1000, 64, 1537], shift=0.5)
+    out = {}
+    try:
+        for la in (1, 0):
+            native.lib().th2d_set_lu_lookahead(la)
+            for prec in (native.PREC_F64, native.PREC_F32):
+                db = bag.DenseBatch(mats, prec).lu()
+                out[(la, prec)] = (db.matrices(), db.piv.download((sum(db.ns),), np.int32))
+    finally:
+        native.lib().th2d_set_lu_lookahead(1)
+    for prec in (native.PREC_F64, native.PREC_F32):
+        (m1, p1), (m0, p0) = out[(1, prec)], out[(0, prec)]
+        assert np.array_equal(p1, p0)
+        for a, b in zip(m1, m0):
+            assert np.array_equal(a, b)
+
+
 def test_lu_logdet_f32_bound(bag, native):
     """FP32-MFMA LU: log|det| within the stated bound of the FP64 value, on
     orders up to 2304 (several 128-column blocks, every panel width).

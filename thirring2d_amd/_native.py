@@ -151,6 +151,7 @@ SIGNATURES = {
     "th2d_gemm_batched": (I, [P, P, I, P, I, I, I, D]),
     "th2d_set_dense_mfma": (I, [I]),
     "th2d_set_lu_panel": (I, [I]),
+    "th2d_set_lu_lookahead": (I, [I]),
     "th2d_copy_f64_to_f32": (I, [P, P, P, L]),
     "th2d_bag_build_masked": (I, [P, P, P, P, C.POINTER(Batch), P, I]),
     "th2d_bag_create": (I, [P, C.POINTER(BagGeom), C.POINTER(P)]),

@@ -265,13 +265,14 @@ template <typename T, int NT>
 __global__ void __launch_bounds__(NT) k_lu_panel_reg(T *base, BatchView bv, int *pivb, int *info,
                                                      int k)
 {
+  constexpr int CH = PCH;
   // double-buffered by column parity: a buffer is rewritten two columns
   // later, after every thread has passed the barrier between
   __shared__ double wbest[2][NT / 64];
   __shared__ int widx[2][NT / 64];
   __shared__ T wrow[2][NT / 64][PNB];  // each wave's candidate row
   __shared__ T drow[2][PNB];      // the diagonal row before the exchange
-  __shared__ T stage[PCH * (PNB + 1)];
+  __shared__ T stage[CH * (PNB + 1)];
   const int b = blockIdx.x;
   const int n = bv.n[b];
   if (k >= n) return;
@@ -283,22 +284,22 @@ __global__ void __launch_bounds__(NT) k_lu_panel_reg(T *base, BatchView bv, int 
   for (int q = 0; q < PR; q++)
 #pragma unroll
     for (int c = 0; c < PNB; c++) a[q][c] = (T)0;
-  // rows arrive through LDS in chunks of PCH, read row-contiguously (16
+  // rows arrive through LDS in chunks of CH, read row-contiguously (16
   // lanes per row) rather than one row per lane
 #pragma unroll
   for (int q = 0; q < PR; q++)
 #pragma unroll
-    for (int s = 0; s < NT / PCH; s++) {
-      const int r0 = q * NT + s * PCH;  // first panel row of the chunk, from k
+    for (int s = 0; s < NT / CH; s++) {
+      const int r0 = q * NT + s * CH;  // first panel row of the chunk, from k
       if (k + r0 >= n) continue;
       __syncthreads();
-      for (int e = tid; e < PCH * PNB; e += NT) {
+      for (int e = tid; e < CH * PNB; e += NT) {
         const int r = e / PNB, c = e - r * PNB, i = k + r0 + r;
         stage[r * (PNB + 1) + c] = (i < n && c < jb) ? A[(long)i * n + k + c] : (T)0;
       }
       __syncthreads();
-      const int r = tid - s * PCH;
-      if (r >= 0 && r < PCH)
+      const int r = tid - s * CH;
+      if (r >= 0 && r < CH)
 #pragma unroll
         for (int c = 0; c < PNB; c++) a[q][c] = stage[r * (PNB + 1) + c];
     }
@@ -394,16 +395,16 @@ __global__ void __launch_bounds__(NT) k_lu_panel_reg(T *base, BatchView bv, int 
 #pragma unroll
   for (int q = 0; q < PR; q++)
 #pragma unroll
-    for (int s = 0; s < NT / PCH; s++) {
-      const int r0 = q * NT + s * PCH;
+    for (int s = 0; s < NT / CH; s++) {
+      const int r0 = q * NT + s * CH;
       if (k + r0 >= n) continue;
       __syncthreads();
-      const int r = tid - s * PCH;
-      if (r >= 0 && r < PCH)
+      const int r = tid - s * CH;
+      if (r >= 0 && r < CH)
 #pragma unroll
         for (int c = 0; c < PNB; c++) stage[r * (PNB + 1) + c] = a[q][c];
       __syncthreads();
-      for (int e = tid; e < PCH * PNB; e += NT) {
+      for (int e = tid; e < CH * PNB; e += NT) {
         const int r2 = e / PNB, c = e - r2 * PNB, i = k + r0 + r2;
         if (i < n && c < jb) A[(long)i * n + k + c] = stage[r2 * (PNB + 1) + c];
       }
@@ -417,7 +418,7 @@ __global__ void __launch_bounds__(NT) k_lu_panel_reg(T *base, BatchView bv, int 
 // waits for another thread.
 template <typename T>
 __global__ void __launch_bounds__(256) k_lu_swap_trsm(T *base, BatchView bv, const int *pivb,
-                                                      int k, int c1)
+                                                      int k, int c1, int sw0, int sw1, int cbase)
 {
   __shared__ T sL[PNB * PNB];
   const int b = blockIdx.y;
@@ -426,8 +427,8 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(T *base, BatchView bv, con
   const int jb = min(PNB, n - k);
   T *A = base + bv.off[b];
   const int *piv = pivb + bv.poff[b];
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c < n && !(c >= k && c < k + jb)) {
+  const int c = cbase + blockIdx.x * 256 + threadIdx.x;
+  if (c < n && c >= sw0 && c < sw1 && !(c >= k && c < k + jb)) {
     for (int j = k; j < k + jb; j++) {
       const int p = piv[j];
       if (p != j) {
@@ -438,7 +439,7 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(T *base, BatchView bv, con
     }
   }
   const int cb = k + PNB, ce = min(c1, n);
-  const int w0 = blockIdx.x * 256;
+  const int w0 = cbase + blockIdx.x * 256;
   if (cb >= ce || w0 + 256 <= cb || w0 >= ce) return;  // the whole workgroup
   for (int e = threadIdx.x; e < jb * jb; e += 256) {
     const int r = e / jb, q = e - r * jb;
@@ -457,6 +458,31 @@ __global__ void __launch_bounds__(256) k_lu_swap_trsm(T *base, BatchView bv, con
       for (int q = 0; q < r; q++) sum -= sL[r * PNB + q] * x[q];
       x[r] = sum;
       A[(long)(k + r) * n + c] = sum;
+    }
+  }
+}
+
+// The row interchanges of the pivots k0..k1-1, in order, on every column
+// outside [k0, k1): the part of a block's interchanges that the look-ahead
+// defers until the previous block's trailing update has left those columns.
+template <typename T>
+__global__ void __launch_bounds__(256) k_lu_swap_out(T *base, BatchView bv, const int *pivb,
+                                                     int k0, int k1)
+{
+  const int b = blockIdx.y;
+  const int n = bv.n[b];
+  if (k0 >= n) return;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= n || (c >= k0 && c < k1)) return;
+  T *A = base + bv.off[b];
+  const int *piv = pivb + bv.poff[b];
+  const int je = min(k1, n);
+  for (int j = k0; j < je; j++) {
+    const int p = piv[j];
+    if (p != j) {
+      const T t0 = A[(long)j * n + c];
+      A[(long)j * n + c] = A[(long)p * n + c];
+      A[(long)p * n + c] = t0;
     }
   }
 }
@@ -729,17 +755,43 @@ BatchView view(const th2d_batch *b) { return BatchView{b->off, b->n, b->poff, b-
 
 template <typename T>
 void launch_update(th2d_ctx *c, bool mf, T *A, BatchView bv, int count, int kk, int K, int r0,
-                   int r1, int c0, int c1, int N)
+                   int r1, int c0, int c1, int N, hipStream_t st = nullptr)
 {
   const int rows = std::min(r1, N) - r0, cols = std::min(c1, N) - c0;
   if (rows <= 0 || cols <= 0) return;
+  if (!st) st = c->stream;
   dim3 g((cols + BN - 1) / BN, (rows + BM - 1) / BM, count);
   if (mf)
-    hipLaunchKernelGGL((k_lu_update<T, true>), g, dim3(256), 0, c->stream, A, bv, kk, K, r0, r1,
-                       c0, c1);
+    hipLaunchKernelGGL((k_lu_update<T, true>), g, dim3(256), 0, st, A, bv, kk, K, r0, r1, c0, c1);
   else
-    hipLaunchKernelGGL((k_lu_update<T, false>), g, dim3(256), 0, c->stream, A, bv, kk, K, r0, r1,
-                       c0, c1);
+    hipLaunchKernelGGL((k_lu_update<T, false>), g, dim3(256), 0, st, A, bv, kk, K, r0, r1, c0, c1);
+}
+
+static int g_lu_lookahead = 0;  // th2d_set_lu_lookahead (measured slower: off)
+
+// The two streams of the look-ahead: panels on a few CUs of every XCD,
+// updates on the rest, so that neither waits for the other's workgroups.
+// CU-mask bit i is taken for the panels when i mod 32 < 8: 8 of every 32
+// CUs, 64 of 256, whether the runtime numbers the CUs XCD by XCD or
+// round-robin over the XCDs.  0 when the streams cannot be made.
+int lu_streams(th2d_ctx *c, hipStream_t *ps, hipStream_t *us)
+{
+  if (c->lu_stream_state == 0) {
+    c->lu_stream_state = -1;
+    const int ncu = c->num_cu > 0 ? c->num_cu : 256;
+    const int words = (ncu + 31) / 32;
+    std::vector<uint32_t> mp(words, 0), mu(words, 0);
+    for (int i = 0; i < ncu; i++) ((i % 32) < 8 ? mp : mu)[i / 32] |= 1u << (i % 32);
+    if (hipExtStreamCreateWithCUMask(&c->lu_ps, words, mp.data()) == hipSuccess &&
+        hipExtStreamCreateWithCUMask(&c->lu_us, words, mu.data()) == hipSuccess &&
+        hipEventCreateWithFlags(&c->lu_ev[0], hipEventDisableTiming) == hipSuccess &&
+        hipEventCreateWithFlags(&c->lu_ev[1], hipEventDisableTiming) == hipSuccess &&
+        hipEventCreateWithFlags(&c->lu_ev[2], hipEventDisableTiming) == hipSuccess)
+      c->lu_stream_state = 1;
+  }
+  *ps = c->lu_ps;
+  *us = c->lu_us;
+  return c->lu_stream_state == 1;
 }
 
 template <typename T>
@@ -750,7 +802,58 @@ int lu_run(th2d_ctx *c, const th2d_batch *bt, T *A, int *piv, int *info)
   const bool mf = use_mfma();
   const int N = bt->max_n, cnt = bt->count;
   constexpr int END = 1 << 30;  // a range bound past every order
-  if (bt->max_n <= PR * PT && g_lu_reg_panel) {
+  hipStream_t ps = nullptr, us = nullptr;
+  if (bt->max_n <= PR * PT && g_lu_reg_panel && g_lu_lookahead && N > LOB &&
+      lu_streams(c, &ps, &us)) {
+    // The same blocks with a look-ahead.  Panel stream: a block's panels,
+    // their interchanges inside the block, its 16-row solves and updates.
+    // Update stream: the block's interchanges on the other columns, U12,
+    // the trailing update of the next block's columns -- after which the
+    // panel stream factors that block -- and then the rest of the trailing
+    // update, which runs beside those panels.
+    hipEvent_t e_start = c->lu_ev[0], e_panel = c->lu_ev[1], e_next = c->lu_ev[2];
+    TH2D_HIP(hipEventRecord(e_start, c->stream));
+    TH2D_HIP(hipStreamWaitEvent(ps, e_start, 0));
+    TH2D_HIP(hipStreamWaitEvent(us, e_start, 0));
+    for (int k = 0; k < N; k += LOB) {
+      const int ce = std::min(k + LOB, N);
+      for (int kk = k; kk < ce; kk += PNB) {
+        const int rows = N - kk;
+        if (rows <= PR * 256)
+          hipLaunchKernelGGL((k_lu_panel_reg<T, 256>), dim3(cnt), dim3(256), 0, ps, A, bv, piv,
+                             info, kk);
+        else if (rows <= PR * 512)
+          hipLaunchKernelGGL((k_lu_panel_reg<T, 512>), dim3(cnt), dim3(512), 0, ps, A, bv, piv,
+                             info, kk);
+        else
+          hipLaunchKernelGGL((k_lu_panel_reg<T, PT>), dim3(cnt), dim3(PT), 0, ps, A, bv, piv,
+                             info, kk);
+        hipLaunchKernelGGL(k_lu_swap_trsm<T>, dim3((ce - k + 255) / 256, cnt), dim3(256), 0, ps,
+                           A, bv, piv, kk, ce, k, ce, k);
+        const int cb = kk + PNB;
+        if (cb < ce) launch_update<T>(c, mf, A, bv, cnt, kk, PNB, cb, END, cb, ce, N, ps);
+      }
+      TH2D_HIP(hipEventRecord(e_panel, ps));
+      TH2D_HIP(hipStreamWaitEvent(us, e_panel, 0));
+      hipLaunchKernelGGL(k_lu_swap_out<T>, dim3((N + 255) / 256, cnt), dim3(256), 0, us, A, bv,
+                         piv, k, ce);
+      if (ce >= N) break;  // (the last block's interchanges on the earlier columns: above)
+      for (int s = k; s < ce; s += NB) {
+        hipLaunchKernelGGL((k_lu_trsm<T, NB>), dim3((N - ce + 255) / 256, cnt), dim3(256), 0, us,
+                           A, bv, s, ce, END);
+        if (s + NB < ce) launch_update<T>(c, mf, A, bv, cnt, s, NB, s + NB, ce, ce, END, N, us);
+      }
+      const int ne = std::min(ce + LOB, N);
+      launch_update<T>(c, mf, A, bv, cnt, k, LOB, ce, END, ce, ne, N, us);
+      TH2D_HIP(hipEventRecord(e_next, us));
+      TH2D_HIP(hipStreamWaitEvent(ps, e_next, 0));
+      if (ne < N) launch_update<T>(c, mf, A, bv, cnt, k, LOB, ce, END, ne, END, N, us);
+    }
+    TH2D_HIP(hipEventRecord(e_panel, ps));
+    TH2D_HIP(hipStreamWaitEvent(c->stream, e_panel, 0));
+    TH2D_HIP(hipEventRecord(e_next, us));
+    TH2D_HIP(hipStreamWaitEvent(c->stream, e_next, 0));
+  } else if (bt->max_n <= PR * PT && g_lu_reg_panel) {
     // blocks of LOB columns.  Inside a block: register panels of PNB
     // columns, each followed by its row interchanges on the other columns,
     // U of its rows for the block's later columns and their update.  After
@@ -771,7 +874,7 @@ int lu_run(th2d_ctx *c, const th2d_batch *bt, T *A, int *piv, int *info)
           hipLaunchKernelGGL((k_lu_panel_reg<T, PT>), dim3(cnt), dim3(PT), 0, c->stream, A, bv,
                              piv, info, kk);
         hipLaunchKernelGGL(k_lu_swap_trsm<T>, dim3((N + 255) / 256, cnt), dim3(256), 0,
-                           c->stream, A, bv, piv, kk, ce);
+                           c->stream, A, bv, piv, kk, ce, 0, END, 0);
         const int cb = kk + PNB;
         if (cb < ce) launch_update<T>(c, mf, A, bv, cnt, kk, PNB, cb, END, cb, ce, N);
       }
@@ -1040,6 +1143,12 @@ extern "C" int th2d_gemm_batched(th2d_ctx *c, void *base, int precision,
 extern "C" int th2d_set_dense_mfma(int on)
 {
   g_use_mfma = on ? 1 : 0;
+  return TH2D_OK;
+}
+
+extern "C" int th2d_set_lu_lookahead(int on)
+{
+  g_lu_lookahead = on ? 1 : 0;
   return TH2D_OK;
 }
 

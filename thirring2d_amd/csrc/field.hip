@@ -69,6 +69,10 @@ extern "C" int th2d_ctx_destroy(th2d_ctx *c)
   for (auto &g : c->cg_graphs) (void)hipGraphExecDestroy(g.exec);
   c->cg_graphs.clear();
   if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
+  if (c->lu_ps) (void)hipStreamDestroy(c->lu_ps);
+  if (c->lu_us) (void)hipStreamDestroy(c->lu_us);
+  for (hipEvent_t e : c->lu_ev)
+    if (e) (void)hipEventDestroy(e);
   buf_release(c->cg64);
   buf_release(c->cg32);
   buf_release(c->cg32b);

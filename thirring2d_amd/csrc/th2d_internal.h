@@ -65,6 +65,10 @@ struct th2d_ctx {
   int cg_reliable = 0;            // mixed solves continue the Krylov sequence across passes
   int res_capacity = -1;          // resident workgroups the device holds (-1: not queried)
   int cg_last_resident = 0;       // the last inner solve: 0 step kernel, 1 resident, 2 both
+  // look-ahead LU (dense.hip): panel and update streams on disjoint CUs
+  int lu_stream_state = 0;  // 0 not tried, 1 made, -1 unavailable
+  hipStream_t lu_ps = nullptr, lu_us = nullptr;
+  hipEvent_t lu_ev[3] = {nullptr, nullptr, nullptr};
   int cg_small = 1;               // FP64 solves of lattices up to 2048 sites in one launch
   long res_spin_limit = -1;       // polls of one resident-kernel wait (-1: the default)
   long cg_sync_fallbacks = 0;     // resident solves that timed out and were re-solved
