@@ -262,6 +262,16 @@ typedef struct {
   int status;
 } th2d_hmc_result;
 
+/* MD steps as graph replays (1, default; 0 off).  When every chain of a
+ * trajectory has at most 2048 sites and the solves are FP64 (so that each
+ * solve is one launch of the small-lattice kernel, th2d_set_cg_small), one
+ * leapfrog step -- gauge step, solve, M chi, force, gauge step -- has no host
+ * decision in it: it is captured once per trajectory as a hipGraph and
+ * launched nsteps times, and the solver statistics are read once at the
+ * end.  The results are those of the step-by-step launches.  Replaces no
+ * reference function (the loop of hmc.c:708-716). */
+int th2d_set_hmc_graph(th2d_ctx *ctx, int on);
+
 /* One trajectory on device-resident fields.  A (eo, FP64) is the current
  * gauge field and is replaced on accept.  The noise comes either from the
  * host (xi, mom, st: natural-order host arrays drawn from the reference's

@@ -416,6 +416,39 @@ def test_cg_small_lattice_kernel(hmc, oracle):
         assert abs(it1 - it0) <= max(2, it0 // 50)
 
 
+def test_md_step_graph_matches_eager(hmc):
+    """Small-lattice FP64 trajectories with every MD step a replay of one
+    captured graph (th2d_set_hmc_graph) give bitwise the results of the
+    step-by-step launches: dH, the actions, accept decisions, solver
+    statistics and fields, over two consecutive trajectories."""
+    from thirring2d_amd import _native as N
+    dims = [(16, 16), (8, 12), (32, 16), (24, 24)]
+    out = {}
+    ctx = hmc.Context.get()
+    try:
+        for on in (1, 0):
+            N.check(N.lib().th2d_set_hmc_graph(ctx.ptr, on), "graph")
+            lats = []
+            for k, (nt, nx) in enumerate(dims):
+                lat = hmc.StaggeredLattice(nt, nx, m=0.2, g=0.6)
+                lat.random_links(70 + k)
+                lats.append(lat)
+            batch = hmc.ChainBatch(lats)
+            rec = []
+            for t in range(2):
+                res = batch.update_gauge(nsteps=12, params=hmc.cg_params(tol_rel=1e-11), seed=9,
+                                         counters=[t] * len(dims), u_accept=[0.3] * len(dims))
+                rec.append([(r.dS, tuple(r.acts), r.accepted, r.cg_iterations, r.cg_solves,
+                             r.max_true_rel_resid, r.status) for r in res])
+            out[on] = (rec, [l.get_links() for l in lats])
+    finally:
+        N.lib().th2d_set_hmc_graph(ctx.ptr, 1)
+    assert out[1][0] == out[0][0]
+    for a, b in zip(out[1][1], out[0][1]):
+        assert np.array_equal(a, b)
+    assert all(r[6] == 0 and r[4] == 13 for tr in out[1][0] for r in tr)
+
+
 def test_cg_resident_wait_timeout_falls_back(hmc, oracle):
     """A bounded wait of the resident solver that gives up (forced from the
     host with a spin limit of 0 polls: every workgroup still exits normally)

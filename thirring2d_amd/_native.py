@@ -115,6 +115,7 @@ SIGNATURES = {
     "th2d_hmc_trajectory": (I, [P, C.POINTER(Geom), C.POINTER(HmcParams), P, P, P, P, D, U64,
                                 U64, C.POINTER(HmcResult)]),
     "th2d_philox4x32_10": (I, [P, L, P, P]),
+    "th2d_set_hmc_graph": (I, [P, I]),
     "th2d_random_links": (I, [P, C.POINTER(Geom), P, U64]),
     "th2d_puregauge_sweep": (I, [P, C.POINTER(Geom), P, P, D, I]),
     "th2d_puregauge_sweeps": (I, [P, C.POINTER(Geom), P, D, I, I, U64, U64]),

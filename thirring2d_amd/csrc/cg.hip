@@ -1307,7 +1307,8 @@ struct SmallInfo {
 };
 
 template <int NT, int SPT>
-__global__ void __launch_bounds__(NT) k_cg_small_full(const SmallFull *chains, SmallInfo *out)
+__global__ void __launch_bounds__(NT) k_cg_small_full(const SmallFull *chains, SmallInfo *out,
+                                                      int accumulate)
 {
   constexpr int NW = NT / 64;
   __shared__ double2 buf[NT * SPT];
@@ -1469,15 +1470,30 @@ __global__ void __launch_bounds__(NT) k_cg_small_full(const SmallFull *chains, S
   for (int k = 0; k < SPT; k++)
     if (threadIdx.x + k * NT < V) C.x[threadIdx.x + k * NT] = x[k];
   if (threadIdx.x == 0) {
-    SmallInfo o;
-    o.iters = it;
-    o.restarts = restarts;
-    o.status = status;
-    o.pad = 0;
-    o.true_rel = bb > 0 ? sqrt(rtrue / bb) : 0.0;
-    o.rr = rr;
-    out[blockIdx.x] = o;
+    const double tr = bb > 0 ? sqrt(rtrue / bb) : 0.0;
+    SmallInfo &o = out[blockIdx.x];
+    if (accumulate) {  // successive solves of one chain: sums, worst, last
+      o.iters += it;
+      o.restarts += restarts;
+      if (o.status == TH2D_OK) o.status = status;
+      o.pad += 1;  // solves
+      o.true_rel = fmax(o.true_rel, tr);
+      o.rr = rr;
+    } else {
+      o.iters = it;
+      o.restarts = restarts;
+      o.status = status;
+      o.pad = 1;
+      o.true_rel = tr;
+      o.rr = rr;
+    }
   }
+}
+
+__global__ void k_small_info_zero(SmallInfo *out, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = SmallInfo{0, 0, TH2D_OK, 0, 0.0, 0.0};
 }
 
 constexpr long kSmallMaxSites = 2048;  // 1024 threads x 2 sites without spills
@@ -1559,17 +1575,20 @@ int small_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   return TH2D_OK;
 }
 
-// th2d_cg_mdagm_batched in FP64 for small lattices: one launch.
-int small_full_solve(th2d_ctx *ctx, int nch, const Geo *G, double **x, const double *const *b,
-                     const double *const *K, const th2d_cg_params *p, th2d_cg_info *infos)
+// th2d_cg_mdagm_batched in FP64 for small lattices: one launch.  Set-up
+// (descriptors into `buf`) and launch are separate so that the HMC
+// trajectory can put the launch into a captured graph.
+int small_full_prepare(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, double *const *x,
+                       const double *const *b, const double *const *K, const th2d_cg_params *p,
+                       SmallFull **dch, SmallInfo **dinf, long *maxV)
 {
-  long maxV = 0;
-  for (int c = 0; c < nch; c++) maxV = std::max(maxV, 2L * G[c].vh);
+  *maxV = 0;
+  for (int c = 0; c < nch; c++) *maxV = std::max(*maxV, 2L * G[c].vh);
   auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
   const size_t cb = al(sizeof(SmallFull) * nch);
-  TH2D_TRY(th2d_buf_reserve(ctx->cg64, cb + sizeof(SmallInfo) * nch));
-  SmallFull *dch = (SmallFull *)ctx->cg64.ptr;
-  SmallInfo *dinf = (SmallInfo *)((char *)ctx->cg64.ptr + cb);
+  TH2D_TRY(th2d_buf_reserve(buf, cb + sizeof(SmallInfo) * nch));
+  *dch = (SmallFull *)buf.ptr;
+  *dinf = (SmallInfo *)((char *)buf.ptr + cb);
   std::vector<SmallFull> ch(nch);
   for (int c = 0; c < nch; c++) {
     memset(&ch[c], 0, sizeof(SmallFull));
@@ -1581,18 +1600,38 @@ int small_full_solve(th2d_ctx *ctx, int nch, const Geo *G, double **x, const dou
     ch[c].tol_abs_rr = p->tol_abs_rr;
     ch[c].max_iter = p->max_iter;
   }
-  TH2D_HIP(hipMemcpyAsync(dch, ch.data(), sizeof(SmallFull) * nch, hipMemcpyHostToDevice,
+  TH2D_HIP(hipMemcpyAsync(*dch, ch.data(), sizeof(SmallFull) * nch, hipMemcpyHostToDevice,
                           ctx->stream));
-  if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k0, ctx->stream));
+  return TH2D_OK;
+}
+
+void small_full_launch(hipStream_t st, int nch, long maxV, const SmallFull *dch, SmallInfo *dinf,
+                       int accumulate)
+{
   if (maxV <= 256)
-    hipLaunchKernelGGL((k_cg_small_full<256, 1>), dim3(nch), dim3(256), 0, ctx->stream, dch, dinf);
+    hipLaunchKernelGGL((k_cg_small_full<256, 1>), dim3(nch), dim3(256), 0, st, dch, dinf,
+                       accumulate);
   else if (maxV <= 512)
-    hipLaunchKernelGGL((k_cg_small_full<256, 2>), dim3(nch), dim3(256), 0, ctx->stream, dch, dinf);
+    hipLaunchKernelGGL((k_cg_small_full<256, 2>), dim3(nch), dim3(256), 0, st, dch, dinf,
+                       accumulate);
   else if (maxV <= 1024)
-    hipLaunchKernelGGL((k_cg_small_full<256, 4>), dim3(nch), dim3(256), 0, ctx->stream, dch, dinf);
+    hipLaunchKernelGGL((k_cg_small_full<256, 4>), dim3(nch), dim3(256), 0, st, dch, dinf,
+                       accumulate);
   else
-    hipLaunchKernelGGL((k_cg_small_full<1024, 2>), dim3(nch), dim3(1024), 0, ctx->stream, dch,
-                       dinf);
+    hipLaunchKernelGGL((k_cg_small_full<1024, 2>), dim3(nch), dim3(1024), 0, st, dch, dinf,
+                       accumulate);
+}
+
+int small_full_solve(th2d_ctx *ctx, int nch, const Geo *G, double *const *x,
+                     const double *const *b, const double *const *K, const th2d_cg_params *p,
+                     th2d_cg_info *infos)
+{
+  SmallFull *dch;
+  SmallInfo *dinf;
+  long maxV;
+  TH2D_TRY(small_full_prepare(ctx, ctx->cg64, nch, G, x, b, K, p, &dch, &dinf, &maxV));
+  if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k0, ctx->stream));
+  small_full_launch(ctx->stream, nch, maxV, dch, dinf, 0);
   TH2D_HIP(hipGetLastError());
   if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k1, ctx->stream));
   const size_t rb = sizeof(SmallInfo) * nch;
@@ -2556,3 +2595,55 @@ extern "C" int th2d_cg_mdagm(th2d_ctx *ctx, const th2d_geom *g, double *x, const
   const float *k32[1] = {K32};
   return th2d_cg_mdagm_batched(ctx, 1, g, xs, bs, ks, K32 ? k32 : nullptr, p, info);
 }
+
+// ---- one MD step of small lattices as a captured graph (hmc.hip) ----------
+namespace th2d {
+bool md_graph_fits(th2d_ctx *ctx, int nch, const Geo *G, const th2d_cg_params *p)
+{
+  return ctx->cg_small && p->precision == TH2D_PREC_F64 && small_fits(nch, G) &&
+         !(ctx->tile_t > 0 && ctx->tile_x > 0);
+}
+
+// Descriptors of the step's solve (chi_c = (M^dag M)^-1 psi_c) and a zeroed
+// per-chain accumulator, into ctx->cg64; the launch itself is enqueued by
+// md_graph_solve (inside the capture).
+int md_graph_prepare(th2d_ctx *ctx, int nch, const Geo *G, double *const *x,
+                     const double *const *b, const double *const *K, const th2d_cg_params *p,
+                     void **state)
+{
+  SmallFull *dch;
+  SmallInfo *dinf;
+  long maxV;
+  TH2D_TRY(small_full_prepare(ctx, ctx->cg64, nch, G, x, b, K, p, &dch, &dinf, &maxV));
+  hipLaunchKernelGGL(k_small_info_zero, dim3((nch + 63) / 64), dim3(64), 0, ctx->stream, dinf, nch);
+  TH2D_HIP(hipGetLastError());
+  state[0] = dch;
+  state[1] = dinf;
+  state[2] = (void *)maxV;
+  return TH2D_OK;
+}
+
+void md_graph_solve(hipStream_t st, int nch, void *const *state)
+{
+  small_full_launch(st, nch, (long)state[2], (const SmallFull *)state[0], (SmallInfo *)state[1], 1);
+}
+
+// The accumulated solver statistics: iterations, solves, worst status and
+// the largest true relative residual per chain.
+int md_graph_stats(th2d_ctx *ctx, int nch, void *const *state, int *iters, int *solves,
+                   int *status, double *max_true_rel)
+{
+  std::vector<SmallInfo> h(nch);
+  TH2D_HIP(hipMemcpyAsync(h.data(), state[1], sizeof(SmallInfo) * nch, hipMemcpyDeviceToHost,
+                          ctx->stream));
+  TH2D_HIP(hipStreamSynchronize(ctx->stream));
+  for (int c = 0; c < nch; c++) {
+    iters[c] = h[c].iters;
+    solves[c] = h[c].pad;
+    status[c] = h[c].status;
+    max_true_rel[c] = h[c].true_rel;
+  }
+  ctx->cg_last_resident = 3;
+  return TH2D_OK;
+}
+}  // namespace th2d

@@ -221,7 +221,55 @@ extern "C" int th2d_hmc_trajectory_batched(th2d_ctx *c, int nch, const th2d_geom
   };
 
   // ---- leapfrog (hmc.c:708-716) ----
-  for (int s = 0; s < h0.nsteps; s++) {
+  // Small lattices in FP64: one MD step (gauge step, the whole solve in one
+  // launch, M chi, force, gauge step) has no host decision in it, so it is
+  // captured once as a graph and replayed nsteps times; the solver's
+  // statistics accumulate on the device and are read once.
+  bool graphed = false;
+  if (c->hmc_graph && th2d::md_graph_fits(c, nch, G.data(), &h0.cg)) {
+    void *gs[3];
+    TH2D_TRY(th2d::md_graph_prepare(c, nch, G.data(), xs.data(), bs.data(), ks.data(), &h0.cg, gs));
+    if (!c->cap_stream) TH2D_HIP(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+      hipStream_t own = c->stream;
+      c->stream = c->cap_stream;  // the site operations enqueue into the capture
+      int rc = run(T_GS, th2d::BOP_GAUGE_STEP);
+      th2d::md_graph_solve(c->cap_stream, nch, gs);
+      if (rc == TH2D_OK) rc = run(T_MCHI, th2d::BOP_DSLASH);
+      if (rc == TH2D_OK) rc = run(T_FORCE, th2d::BOP_FORCE);
+      if (rc == TH2D_OK) rc = run(T_GS, th2d::BOP_GAUGE_STEP);
+      c->stream = own;
+      const hipError_t e = hipStreamEndCapture(c->cap_stream, &graph);
+      TH2D_TRY(rc);
+      TH2D_HIP(e);
+      hipGraphExec_t ex = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      TH2D_HIP(ei);
+      hipError_t el = hipSuccess;
+      for (int s = 0; s < h0.nsteps && el == hipSuccess; s++) el = hipGraphLaunch(ex, c->stream);
+      std::vector<int> its(nch), nsol(nch), st(nch);
+      std::vector<double> tr(nch);
+      const int rs = el == hipSuccess ? th2d::md_graph_stats(c, nch, gs, its.data(), nsol.data(),
+                                                              st.data(), tr.data())
+                                      : TH2D_EHIP;
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipGraphExecDestroy(ex);
+      TH2D_HIP(el);
+      TH2D_TRY(rs);
+      for (int k = 0; k < nch; k++) {
+        R[k].cg_iterations += its[k];
+        R[k].cg_solves += nsol[k];
+        R[k].max_true_rel_resid = std::max(R[k].max_true_rel_resid, tr[k]);
+        if (st[k] != TH2D_OK && R[k].status == TH2D_OK) R[k].status = st[k];
+      }
+      graphed = true;
+    } else {
+      (void)hipGetLastError();
+    }
+  }
+  for (int s = 0; s < h0.nsteps && !graphed; s++) {
     TH2D_TRY(run(T_GS, th2d::BOP_GAUGE_STEP));
     TH2D_TRY(solve());
     TH2D_TRY(run(T_MCHI, th2d::BOP_DSLASH));
@@ -269,6 +317,13 @@ extern "C" int th2d_hmc_trajectory_batched(th2d_ctx *c, int nch, const th2d_geom
     if (R[k].status != TH2D_OK && worst == TH2D_OK) worst = R[k].status;
   }
   return worst;
+}
+
+extern "C" int th2d_set_hmc_graph(th2d_ctx *c, int on)
+{
+  if (!c) return TH2D_EARG;
+  c->hmc_graph = on ? 1 : 0;
+  return TH2D_OK;
 }
 
 extern "C" int th2d_hmc_trajectory(th2d_ctx *c, const th2d_geom *geo, const th2d_hmc_params *hp,

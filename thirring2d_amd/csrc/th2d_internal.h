@@ -69,6 +69,7 @@ struct th2d_ctx {
   int lu_stream_state = 0;  // 0 not tried, 1 made, -1 unavailable
   hipStream_t lu_ps = nullptr, lu_us = nullptr;
   hipEvent_t lu_ev[3] = {nullptr, nullptr, nullptr};
+  int hmc_graph = 1;              // small-lattice FP64 MD steps as graph replays
   int cg_small = 1;               // FP64 solves of lattices up to 2048 sites in one launch
   long res_spin_limit = -1;       // polls of one resident-kernel wait (-1: the default)
   long cg_sync_fallbacks = 0;     // resident solves that timed out and were re-solved
@@ -363,6 +364,17 @@ int fetch_doubles(th2d_ctx *ctx, double *host, const double *dev, int n);
 // Device address of the static scratch array of the allocation-free LU
 // inverse on the current device, looked up once per device (dense.hip).
 int noalloc_scratch_init(int device);
+// One MD step of a batch of small lattices as a captured graph (cg.hip):
+// whether the batch qualifies, the solve's set-up (state: 3 pointers), its
+// launch into a stream (accumulating per-chain statistics on the device)
+// and the statistics after the replays.
+bool md_graph_fits(th2d_ctx *ctx, int nch, const Geo *G, const th2d_cg_params *p);
+int md_graph_prepare(th2d_ctx *ctx, int nch, const Geo *G, double *const *x,
+                     const double *const *b, const double *const *K, const th2d_cg_params *p,
+                     void **state);
+void md_graph_solve(hipStream_t st, int nch, void *const *state);
+int md_graph_stats(th2d_ctx *ctx, int nch, void *const *state, int *iters, int *solves,
+                   int *status, double *max_true_rel);
 
 // Batched site operations over a device table of nch descriptors covering
 // nblk blocks of 256 sites (table built by bop_finish).
