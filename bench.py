@@ -37,11 +37,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 # measured HBM bytes per launch of the dominant kernel (FETCH_SIZE + WRITE_SIZE,
-# copy-calibrated): k_cg_resident at 16 chains
+# copy-calibrated): k_cg_resident at 16 thermalized chains, 128 x dt = 1/128
 # (profiles/k_cg_resident_hmc256_16chains_mixed.md), k_cg_step at 32 chains
 # (profiles/k_cg_step_hmc256_32chains_mixed.md)
 # key: (lattice size, chains, resident solver)
-TRAFFIC_BYTES = {(256, 32, False): 199.6e6, (256, 16, True): 112.8e6}
+TRAFFIC_BYTES = {(256, 32, False): 199.6e6, (256, 16, True): 103.1e6}
 
 
 def parse():
@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--no-c3", action="store_true", help="skip the 128^2 force leg")
     ap.add_argument("--oracle-apply", action="store_true",
                     help="time the oracle's fm_mul at 256^2 on this host and exit")
-    ap.add_argument("--c4-steps", type=int, default=3,
+    ap.add_argument("--c4-steps", type=int, default=50,
                     help="trajectories per chain of the 256-chain uneven batch (0 = skip)")
     ap.add_argument("--c5-proposals", type=int, default=150,
                     help="bag proposals per matrix in the dense leg (0 = skip the leg)")
@@ -197,21 +197,29 @@ def mixed_vs_fp64(H, N, lats, nsteps, tau=1e-10):
             "bound": "4 tau |xi|^2 sqrt(m^2+4)/m per chain, tau = 1e-10"}
 
 
-def c4_leg(H, N, steps):
+def c4_leg(H, N, steps, nsteps=64):
     """256 chains, L uniform on {32,48,64,96,128} (seed 4004), g ~ U[0.2,1],
-    m = 0.1, mixed CG to 1e-10, 10 x dt = 0.1, autotuned tiles."""
+    m = 0.1, mixed CG to 1e-10, autotuned tiles; every chain starts as
+    hmc.c:927-929 does (A = 0, 100 pure-gauge sweeps) and runs `steps`
+    trajectories of nsteps x dt = 1/nsteps."""
     rng = np.random.default_rng(4004)
     Ls = rng.choice([32, 48, 64, 96, 128], size=256)
     gs = rng.uniform(0.2, 1.0, size=256)
     lats = []
     for c, (Lc, gc) in enumerate(zip(Ls, gs)):
         lat = H.StaggeredLattice(int(Lc), int(Lc), m=0.1, g=float(gc))
-        lat.random_links(40040 + c)
+        lat.thermalize_pure_gauge(100, seed=40040 + c)
         lats.append(lat)
-    dt, res, kms, kn, kby = run_batch(H, N, lats, list(gs), N.PREC_MIXED, steps, 1)
+    dt, res, kms, kn, kby = run_batch(H, N, lats, list(gs), N.PREC_MIXED, steps, 1,
+                                      nsteps=nsteps)
     sites = int(np.sum(Ls.astype(np.int64) ** 2))
+    dH = np.array([r[0] for r in res])
     return {"value": 256 * steps / dt, "unit": "traj/s", "chains": 256, "sites": sites,
-            "trajectories": 256 * steps, "sizes": "L in {32,48,64,96,128}, seed 4004",
+            "trajectories": 256 * steps, "nsteps": nsteps, "dt": 1.0 / nsteps,
+            "sizes": "L in {32,48,64,96,128}, seed 4004",
+            "acceptance": float(np.mean([r[1] for r in res])),
+            "exp_minus_dH_mean": float(np.mean(np.exp(-np.minimum(dH, 50.0)))),
+            "dH_median": float(np.median(dH)),
             "max_true_rel_resid": max(r[4] for r in res),
             "cg_kernel_GBs": kby / (kms * 1e-3) / 1e9 if kms > 0 else 0.0}
 

@@ -188,8 +188,9 @@ typedef struct {
   double ratio;
 } th2d_bag_move;
 
-/* count configurations: geoms (host) and their fields concatenated (host
- * int32, natural order, V_b values each); B of each by batched LU. */
+/* count configurations (1..65535): geoms (host) and their fields
+ * concatenated (host int32, natural order, V_b values each); B of each by
+ * batched LU. */
 int th2d_bag_batch_create(th2d_ctx *ctx, int count, const th2d_bag_geom *geoms, const int *fields,
                           th2d_bag_batch **out);
 int th2d_bag_batch_destroy(th2d_bag_batch *b);

@@ -251,28 +251,39 @@ __global__ void __launch_bounds__(256) k_bmc_snap(BView v, int pass)
 }
 
 // B += U C W (C = Sinv in pass 0, -(B1_OO)^-1 in pass 1), rank 1 or 2.
+// A thread owns column j of a band of BMC_ROWS rows: it forms (C W)_j once,
+// issues the band's loads of B together, then its stores, so each row's
+// accesses are contiguous across the wavefront and U is a broadcast.
+constexpr int BMC_ROWS = 16;
+
 __global__ void __launch_bounds__(256) k_bmc_update(BView v, int pass)
 {
   const int b = blockIdx.y;
   const BMove &m = v.mv[b];
   if (!pass_active(m, pass)) return;
-  const long V = v.V[b];
+  const int V = v.V[b];
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i0 = blockIdx.z * BMC_ROWS;
+  if (j >= V || i0 >= V) return;
+  const int nr = min(BMC_ROWS, V - i0);
   const int k = pass == 0 ? m.nr : m.no;
   const double *C = pass == 0 ? m.Sinv : m.Cocc;
-  double *B = v.B + v.boff[b];
-  const double *U = v.U + 2 * v.foff[b], *W = v.W + 2 * v.foff[b];
-  const long n = V * V;
-  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-       e += (long)gridDim.x * blockDim.x) {
-    const long i = e / V, j = e - i * V;
-    double acc = 0.0;
-    for (int a = 0; a < k; a++) {
-      double uc = 0.0;
-      for (int c = 0; c < k; c++) uc += C[a * k + c] * W[(long)c * V + j];
-      acc += U[(long)a * V + i] * uc;
+  double *__restrict__ B = v.B + v.boff[b] + (long)i0 * V + j;
+  const double *__restrict__ U = v.U + 2 * v.foff[b] + i0;
+  const double *__restrict__ W = v.W + 2 * v.foff[b];
+  const double w0 = W[j], w1 = k == 2 ? W[V + j] : 0.0;
+  const double cw0 = k == 2 ? C[0] * w0 + C[1] * w1 : C[0] * w0;
+  const double cw1 = k == 2 ? C[2] * w0 + C[3] * w1 : 0.0;
+  double bv[BMC_ROWS];
+#pragma unroll
+  for (int r = 0; r < BMC_ROWS; r++)
+    if (r < nr) bv[r] = B[(long)r * V];
+#pragma unroll
+  for (int r = 0; r < BMC_ROWS; r++)
+    if (r < nr) {
+      const double acc = k == 2 ? U[r] * cw0 + U[V + r] * cw1 : U[r] * cw0;
+      B[(long)r * V] = bv[r] + acc;
     }
-    B[e] += acc;
-  }
 }
 
 // Pass 0: the R diagonal of B was 1 and is not part of B1 (B1 = B +
@@ -384,7 +395,8 @@ extern "C" int th2d_bag_batch_destroy(th2d_bag_batch *s)
 extern "C" int th2d_bag_batch_create(th2d_ctx *c, int count, const th2d_bag_geom *geoms,
                                      const int *fields, th2d_bag_batch **out)
 {
-  if (!c || count < 1 || !geoms || !fields || !out) return TH2D_EARG;
+  // count is a grid dimension of every batch launch
+  if (!c || count < 1 || count > 65535 || !geoms || !fields || !out) return TH2D_EARG;
   for (int b = 0; b < count; b++) TH2D_TRY(check_bag_geom(&geoms[b]));
   th2d_bag_batch *s = new th2d_bag_batch();
   s->ctx = c;
@@ -459,8 +471,8 @@ extern "C" int th2d_bag_batch_sweep(th2d_bag_batch *s, int nsteps, unsigned long
   const BView v = s->view();
   const unsigned g1 = (unsigned)((s->count + 63) / 64);
   const dim3 gv((unsigned)((s->maxV + 255) / 256), (unsigned)s->count);
-  const long n2 = (long)s->maxV * s->maxV;
-  const dim3 gu((unsigned)std::min<long>(2048, (n2 + 255) / 256), (unsigned)s->count);
+  const dim3 gu((unsigned)((s->maxV + 255) / 256), (unsigned)s->count,
+                (unsigned)((s->maxV + BMC_ROWS - 1) / BMC_ROWS));
   for (int k = 0; k < nsteps; k++) {
     th2d_bag_move *tr = trace ? trace + (size_t)k * s->count : nullptr;
     hipLaunchKernelGGL(k_bmc_propose, dim3(g1), dim3(64), 0, c->stream, v, seed,
