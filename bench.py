@@ -74,6 +74,9 @@ def parse():
                     help="bag proposals per matrix in the dense leg (0 = skip the leg)")
     ap.add_argument("--c1-steps", type=int, default=40,
                     help="16^2 FP64 trajectories (0 = skip)")
+    ap.add_argument("--only-legs", default="", metavar="LIST",
+                    help="run only these legs, comma separated, and exit: c1, c4, "
+                         "bag-driver (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB five times and exit (scale of "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -340,10 +343,7 @@ def c5_leg(N, proposals, device_steps=1000):
     from thirring2d_amd import bag as Bm
     from thirring2d_amd.hmc import Context
     import ctypes as C
-    rng = np.random.default_rng(5005)
-    Ls = rng.choice(np.arange(16, 49, 2), size=64)
-    lats = [Bm.BagLattice(int(L), int(L), m=0.1) for L in Ls]
-    fields = [_bag_field(rng, int(L), rng.uniform(0.1, 0.5)) for L in Ls]
+    Ls, lats, fields = c5_batch()
     ctx = Context.get()
     lib = N.lib()
     out = {"matrices": 64, "orders": [int(L) ** 2 for L in Ls][:4] + ["..."],
@@ -419,6 +419,36 @@ def c5_leg(N, proposals, device_steps=1000):
     return out
 
 
+def c5_batch():
+    """The C5 configurations: 64 bag lattices, L uniform on even {16..48}
+    (seed 5005), m = 0.1, occupancy rho ~ U[0.1, 0.5]."""
+    from thirring2d_amd import bag as Bm
+    rng = np.random.default_rng(5005)
+    Ls = rng.choice(np.arange(16, 49, 2), size=64)
+    lats = [Bm.BagLattice(int(L), int(L), m=0.1) for L in Ls]
+    fields = [_bag_field(rng, int(L), rng.uniform(0.1, 0.5)) for L in Ls]
+    return Ls, lats, fields
+
+
+def only_legs(names):
+    """The legs named in `names`, alone, as one JSON line."""
+    from thirring2d_amd import _native as N
+    from thirring2d_amd import hmc as H
+    out = {}
+    for name in names:
+        if name == "c1":
+            out["c1_16x16_fp64"] = c1_leg(H, N, 40)
+        elif name == "c4":
+            N.lib().th2d_set_cg_autotune(H.Context.get().ptr, 1)
+            out["c4_256_chains_uneven"] = c4_leg(H, N, 50)
+        elif name == "bag-driver":
+            _, lats, fields = c5_batch()
+            out["bag_driver"] = bag_driver_leg(N, lats, fields, 1000)
+        else:
+            raise SystemExit(f"bench.py: unknown leg {name!r}")
+    print(json.dumps(out), flush=True)
+
+
 def bag_driver_leg(N, lats, fields, steps):
     """All 64 matrices through `steps` proposals each on the device
     (th2d_bag_batch_sweep: one proposal per matrix per step, no host
@@ -431,15 +461,21 @@ def bag_driver_leg(N, lats, fields, steps):
     _, _, _, acc0 = batch.state()
     lib = N.lib()
     lib.th2d_timer_start(batch.ctx.ptr)
-    batch.sweep(steps, seed=5, counter=1000)
+    moves = batch.sweep(steps, seed=5, counter=1000, trace=True)
     ms = C.c_float()
     lib.th2d_timer_stop(batch.ctx.ptr, C.byref(ms))
     la, sg, _, acc = batch.state()
     batch.refresh()
     la2, sg2, _, _ = batch.state()
     n = len(lats) * steps
+    # an accepted change rewrites B once per pass (freeing, then occupying):
+    # V^2 doubles read and written
+    vv = [float(l.V) ** 2 for l in lats]
+    nbytes = sum(16.0 * vv[b] * ((mv.nr > 0) + (mv.no > 0))
+                 for step in moves for b, mv in enumerate(step) if mv.accepted)
     return {"matrices": len(lats), "proposals": n, "accepted": int(np.sum(acc - acc0)),
             "ms": ms.value, "proposals_per_s": n / (ms.value * 1e-3),
+            "B_update_bytes": nbytes, "B_update_GBs_whole_sweep": nbytes / (ms.value * 1e-3) / 1e9,
             "logdet_drift_max_abs": float(np.max(np.abs(la - la2))),
             "sign_changes_on_refresh": int(np.sum(sg != sg2))}
 
@@ -549,6 +585,9 @@ def main():
         return
     if args.dslash_sweep:
         dslash_sweep()
+        return
+    if args.only_legs:
+        only_legs([x for x in args.only_legs.split(",") if x])
         return
     if args.dense_only:
         from thirring2d_amd import _native as N

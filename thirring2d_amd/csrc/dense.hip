@@ -219,13 +219,14 @@ __global__ void __launch_bounds__(256) k_lu_panel(T *base, BatchView bv, int *pi
 
 // Panel of width PNB held in registers: one block of NT threads per matrix
 // (NT = 256, 512 or 1024, the fewest that hold the rows left), row
-// k + t + q*NT of the panel in thread t, slot q < PR (orders up to PR*PT).  Per column, with one barrier: each wave finds its largest |a_ij|
-// (a max by DPP moves, then the smallest row attaining it by ballots, so
-// ties go to the smaller row as in k_lu_panel) and posts it with that row's
-// panel part to LDS, and the thread holding the diagonal row posts that
-// row; after the barrier every thread picks the pivot from the 16 posts and
-// does the exchange, the scaling of the column and the rank-1 update of the
-// panel's later columns on registers.  Only the panel's columns are swapped
+// k + t + q*NT of the panel in thread t, slot q < PR (orders up to PR*PT).
+// Per column, with one barrier: each wave finds its largest |a_ij| (a max
+// by DPP moves, then the smallest row attaining it by ballots, so ties go
+// to the smaller row as in k_lu_panel) and posts it with that row's panel
+// part to LDS, and the thread holding the diagonal row posts that row;
+// after the barrier every wave reduces the NT/64 posts to the pivot (DPP
+// moves again) and every thread does the exchange, the scaling of the
+// column and the rank-1 update of the panel's later columns on registers.  Only the panel's columns are swapped
 // here; k_lu_swap_trsm applies the panel's interchanges to the other columns
 // afterwards, in order, so the factors equal those of whole-row swaps.
 constexpr int PNB = 16;   // register panel width
@@ -259,6 +260,29 @@ __device__ __forceinline__ double wave_max(double v)
   const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, 63);
   const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), 63);
   return __longlong_as_double(((long long)hi << 32) | lo);
+}
+
+// Maximum (minimum) over lanes 0..15 by DPP moves within the first row of
+// the wave, read from lane 0.
+__device__ __forceinline__ double row16_max(double v)
+{
+  v = fmax(v, dpp_keep<0xB1, 0xf>(v));   // quad_perm [1,0,3,2]
+  v = fmax(v, dpp_keep<0x4E, 0xf>(v));   // quad_perm [2,3,0,1]
+  v = fmax(v, dpp_keep<0x141, 0xf>(v));  // row_half_mirror
+  v = fmax(v, dpp_keep<0x140, 0xf>(v));  // row_mirror
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, 0);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), 0);
+  return __longlong_as_double(((long long)hi << 32) | lo);
+}
+
+__device__ __forceinline__ int row16_min(int v)
+{
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xf, 0xf, false));
+  return __builtin_amdgcn_readlane(v, 0);
 }
 
 template <typename T, int NT>
@@ -347,18 +371,19 @@ __global__ void __launch_bounds__(NT) k_lu_panel_reg(T *base, BatchView bv, int 
       for (int c = 0; c < PNB; c++) drow[par][c] = a[0][c];
     }
     __syncthreads();
-    double bv2 = wbest[par][0];
-    int bi = widx[par][0], bw = 0;
-#pragma unroll
-    for (int w = 1; w < NT / 64; w++) {
-      const double v2 = wbest[par][w];
-      const int i2 = widx[par][w];
-      if (v2 > bv2 || (v2 == bv2 && i2 < bi)) {
-        bv2 = v2;
-        bi = i2;
-        bw = w;
-      }
+    // every wave reduces the NT/64 posts itself: lane w reads post w, then
+    // the largest value and the smallest row at it, by DPP moves over the
+    // first row of 16 lanes (two LDS reads per wave instead of a scan of
+    // every post by every thread)
+    double pv2 = -2.0;
+    int pi2 = 0x7fffffff;
+    if (lane < NT / 64) {
+      pv2 = wbest[par][lane];
+      pi2 = widx[par][lane];
     }
+    const double bv2 = row16_max(pv2);
+    const int bi = row16_min(pv2 == bv2 ? pi2 : 0x7fffffff);
+    const int bw = __builtin_ctzll(__ballot(lane < NT / 64 && pv2 == bv2 && pi2 == bi));
     const int p = bi;  // >= gj: row gj itself is a candidate
     if (tid == 0) {
       piv[gj] = p;
