@@ -504,6 +504,37 @@ def dslash_bw(H, N, size, nrhs, precision, reps=50):
     return per, bytes_ / (per * 1e-3) / 1e9
 
 
+def dslash_batched_bw(H, N, size, chains, reps=50):
+    """Algorithmic GB/s of th2d_dslash_batched (FP64): `chains` lattices of
+    size^2, each with its own links, in one launch; per site one vector in,
+    one out and two links."""
+    import ctypes as C
+    V = size * size
+    lats, ins, outs = [], [], []
+    L = N.lib()
+    for c in range(chains):
+        lat = H.StaggeredLattice(size, size, m=0.05, g=0.5)
+        lat.random_links(1002 + c)
+        src, dst = H.DeviceArray(16 * V), H.DeviceArray(16 * V)
+        L.th2d_memset(lat.ctx.ptr, src.p, 0, 16 * V)
+        lats.append(lat)
+        ins.append(src)
+        outs.append(dst)
+    geoms = (N.Geom * chains)(*[l.geom for l in lats])
+    P = C.c_void_p * chains
+    po, pi, pk = P(*[d.p for d in outs]), P(*[d.p for d in ins]), P(*[l.K.p for l in lats])
+    ctx = lats[0].ctx.ptr
+    for _ in range(3):
+        N.check(L.th2d_dslash_batched(ctx, chains, geoms, po, pi, pk, 0), "dslash_batched")
+    L.th2d_timer_start(ctx)
+    for _ in range(reps):
+        L.th2d_dslash_batched(ctx, chains, geoms, po, pi, pk, 0)
+    ms = C.c_float()
+    L.th2d_timer_stop(ctx, C.byref(ms))
+    per = ms.value / reps
+    return per, chains * V * 64 / (per * 1e-3) / 1e9
+
+
 def dslash_sweep():
     """Algorithmic bandwidth of th2d_dslash from 256^2 to 4096^2 (FP64 and
     FP32, one RHS; 8 RHS at 256^2): the sizes above ~2048^2 (FP64 vectors
@@ -699,6 +730,10 @@ def main():
                                       (4096, 1, N.PREC_F32, "4096_f32")]:
             ms, gbs = dslash_bw(H, N, sz, nrhs, prec)
             d[tag] = {"ms": ms, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS}
+        ms, gbs = dslash_batched_bw(H, N, 256, 16)
+        d["256x16chains_f64"] = {"ms": ms, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS,
+                                 "note": "th2d_dslash_batched: 16 lattices of 256^2, "
+                                         "each with its own links, in one launch"}
         out["dslash"] = d
     if args.c1_steps:
         out["c1_16x16_fp64"] = c1_leg(H, N, args.c1_steps)

@@ -127,6 +127,14 @@ int th2d_gauge_step(th2d_ctx *ctx, const th2d_geom *g, double *A, const double *
  * after the other (stride 2V) and share the link reads. */
 int th2d_dslash(th2d_ctx *ctx, const th2d_geom *g, void *out, const void *in, const void *K,
                 int dagger, int precision, int nrhs);
+/* The FP64 operator on every lattice of a batch in one launch: out[c] =
+ * M in[c] (dagger = 0) or M^dag in[c] (dagger = 1) with links K[c], for
+ * nch lattices of any sizes (geoms on the host, the pointer arrays on the
+ * host, the vectors and links on the device, eo order as th2d_dslash).  The
+ * reference applies fm_mul (hmc.c:123-184) to one lattice at a time; this
+ * is that apply for many independent chains. */
+int th2d_dslash_batched(th2d_ctx *ctx, int nch, const th2d_geom *geoms, double *const *out,
+                        const double *const *in, const double *const *K, int dagger);
 
 /* out = M^dag M in (FP64). */
 int th2d_mdagm(th2d_ctx *ctx, const th2d_geom *g, double *out, const double *in, const double *K,

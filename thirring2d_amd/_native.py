@@ -105,6 +105,7 @@ SIGNATURES = {
     "th2d_fold_links": (I, [P, C.POINTER(Geom), P, P, P]),
     "th2d_gauge_step": (I, [P, C.POINTER(Geom), P, P, D, P, P]),
     "th2d_dslash": (I, [P, C.POINTER(Geom), P, P, P, I, I, I]),
+    "th2d_dslash_batched": (I, [P, I, C.POINTER(Geom), P, P, P, I]),
     "th2d_mdagm": (I, [P, C.POINTER(Geom), P, P, P, P]),
     "th2d_cg_mdagm": (I, [P, C.POINTER(Geom), P, P, P, P, C.POINTER(CgParams),
                           C.POINTER(CgInfo)]),

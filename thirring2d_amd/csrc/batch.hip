@@ -154,3 +154,34 @@ int bop_launch(th2d_ctx *c, int op, const BOp *dev, int nch, long nblk)
 }
 
 }  // namespace th2d
+
+// out_c = M in_c (dagger 0) or M^dag in_c (dagger 1) for every chain c of a
+// batch of lattices of any sizes, in one launch (FP64, eo order, as
+// th2d_dslash).  The descriptor table is uploaded only when it differs from
+// the previous call's on this context.
+extern "C" int th2d_dslash_batched(th2d_ctx *c, int nch, const th2d_geom *geoms,
+                                   double *const *out, const double *const *in,
+                                   const double *const *K, int dagger)
+{
+  if (!c || nch < 1 || !geoms || !out || !in || !K) return TH2D_EARG;
+  std::vector<BOp> ops(nch);
+  for (int k = 0; k < nch; k++) {
+    TH2D_TRY(th2d_check_geom(&geoms[k]));
+    if (!out[k] || !in[k] || !K[k] || out[k] == in[k]) return TH2D_EARG;
+    memset(&ops[k], 0, sizeof(BOp));  // padding too: the bytes are the cache key
+    ops[k].G = make_geo(&geoms[k]);
+    ops[k].bc = Bc{geoms[k].anti_t, geoms[k].anti_x};
+    ops[k].d0 = out[k];
+    ops[k].d1 = (double *)in[k];
+    ops[k].d2 = (double *)K[k];
+    ops[k].s1 = dagger ? 1.0 : 0.0;
+  }
+  const long nblk = th2d::bop_finish(ops);
+  const size_t bytes = sizeof(BOp) * ops.size();
+  const unsigned char *raw = (const unsigned char *)ops.data();
+  if (c->dsl_key.size() != bytes || memcmp(c->dsl_key.data(), raw, bytes) != 0) {
+    TH2D_TRY(th2d::bop_upload(c, c->dsl_tab, ops));
+    c->dsl_key.assign(raw, raw + bytes);
+  }
+  return th2d::bop_launch(c, th2d::BOP_DSLASH, (const BOp *)c->dsl_tab.ptr, nch, nblk);
+}

@@ -85,6 +85,7 @@ extern "C" int th2d_ctx_destroy(th2d_ctx *c)
   buf_release(c->misc);
   buf_release(c->bpart);
   for (auto &b : c->btab) buf_release(b);
+  buf_release(c->dsl_tab);
   (void)hipEventDestroy(c->t0);
   (void)hipEventDestroy(c->t1);
   (void)hipEventDestroy(c->k0);

@@ -53,6 +53,8 @@ struct th2d_ctx {
   th2d_buf cg64, cg32, cg32b, cgmix, red, redh, hmc, force, bag, misc;  // cg32b: step-kernel
                                                                        // part of a split batch
   th2d_buf bpart, btab[4];  // batched site operations: partials, descriptor tables
+  th2d_buf dsl_tab;                    // th2d_dslash_batched: its descriptor table,
+  std::vector<unsigned char> dsl_key;  // uploaded again only when it changes
   // tile configuration of the persistent solver (0 = autotuned default)
   int tile_t = 0, tile_x = 0;
   int cg_max_blocks = 0;  // 0 = one workgroup per tile
