@@ -689,17 +689,18 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   using V2 = c2<T>;
   static_assert(sizeof(V2) == 8, "the resident solver is FP32");
   constexpr int TT = S::TT, TX = S::TX, NT = S::NT, W2 = S::W2, W1 = S::W1;
-  constexpr int E2 = S::E2, E1 = S::E1, NO = S::NO, N1 = S::N1;
+  constexpr int E2 = S::E2, E1 = S::E1, NO = S::NO;
   constexpr int LDS_D = (3 * E2 + E1) * (int)sizeof(V2) / 8;
   __shared__ double rlds[LDS_D + 40];
   V2 *R2 = (V2 *)rlds, *Y1 = R2 + E2, *LKt = Y1 + E1, *LKx = LKt + E2;
   double *red = rlds + LDS_D;
   const int tid = threadIdx.x;
-  // a wave holds one row of the region: element q of a thread lies at row
-  // wrow + 8 q (uniform in the wave, kept in a scalar register so that the
-  // row tests branch on the scalar unit) and column lane
-  static_assert(TX == 64, "a wave covers one row of a region");
-  const int wrow = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  // a wave holds NO consecutive rows of the region: element q of a thread
+  // lies at row wrow + q (uniform in the wave, kept in a scalar register so
+  // that the row tests branch on the scalar unit) and column lane
+  static_assert(TX == 64, "a wave covers whole rows of a region");
+  static_assert(NO * (NT / 64) == TT, "the waves' rows tile the region");
+  const int wrow = __builtin_amdgcn_readfirstlane(tid >> 6) * NO, lane = tid & 63;
   const int tau = ctl.wg_tile[blockIdx.x];  // uniform: idle workgroups leave at once
   if (tau < 0) return;
   const CgTile tl = a.tiles[tau];
@@ -756,7 +757,7 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   // (a region larger than what is left of it) are halo, reloaded each time
   // (byte offset of element q in a chain vector)
   auto oi_of = [&](int q) {
-    const int lt = wrow + q * (NT / TX), lx = lane;
+    const int lt = wrow + q, lx = lane;
     return (unsigned)eo_index(G, wrapn(tl.t0 + lt, G.nt), wrapn(tl.x0 + lx, G.nx)) *
            (unsigned)sizeof(V2);
   };
@@ -765,7 +766,7 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   const V2 zero = cmk<V2>((T)0, (T)0);
 #pragma unroll
   for (int q = 0; q < NO; q++) {
-    const int lt = wrow + q * (NT / TX), lx = lane;
+    const int lt = wrow + q, lx = lane;
     const unsigned oi = oi_of(q);
     if (lt < tte && lx < txe) {
       inl |= 1u << q;
@@ -831,7 +832,7 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
 #pragma unroll
       for (int q = 0; q < NO; q++) {
         if ((inl >> q) & 1u) continue;
-        const int lt = wrow + q * (NT / TX), lx = lane;
+        const int lt = wrow + q, lx = lane;
         const unsigned so = src_at(lt, lx);
         r[q] = w[q] = s[q] = zero;
         if (so != kNoLoad) {
@@ -842,7 +843,7 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       }
 #pragma unroll
       for (int q = 0; q < NO; q++) {
-        const int lt = wrow + q * (NT / TX), lx = lane;
+        const int lt = wrow + q, lx = lane;
         const bool own = (inl >> q) & 1u;
         const V2 sn = cadd(w[q], cscale(s[q], tb));
         const V2 rn = csub(r[q], cscale(sn, ta));
@@ -869,60 +870,100 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
 #pragma unroll
       for (int q = 0; q < NO; q++)
         if ((pub >> q) & 1u) {
-          const int lt = wrow + q * (NT / TX), lx = lane;
+          const int lt = wrow + q, lx = lane;
           const unsigned po = pub_at(lt, lx);
           M.st2(o_r[0], po, r[q]);
           M.st2(o_s[0], po, s[q]);
         }
     }
     __syncthreads();
-    // 3b. y = M r_{j+1} on the 1-halo (LDS only; not unrolled, which
-    // would hoist every element's loads into registers)
+    // 3b. y = M r_{j+1} on the 1-halo (LDS only).  A wave takes the NO
+    // rows wrow + 1 .. wrow + NO of the 1-halo frame at column lane + 1;
+    // going down the rows, r(t-1), r(t) and the link K_t(t-1) come from the
+    // previous row's registers (6 LDS loads per element instead of 9).  The
+    // loop is not unrolled, which would hoist every row's loads into
+    // registers.  Then the frame's first and last rows and columns, at most
+    // one element per thread.
+    {
+      int c = (wrow + 2) * W2 + (lane + 2);  // R2 index of 1-halo (wrow + 1, lane + 1)
+      V2 rtm = R2[c - W2], r0 = R2[c], ktd = LKt[c - W2];
+      V2 *yo = Y1 + (wrow + 1) * W1 + (lane + 1);
 #pragma unroll 1
-    for (int q = 0; q < N1; q++) {
-      const int e = tid + q * NT;
-      if (!(E1 % NT == 0 || e < E1)) continue;
-      const int ft = e / W1, fx = e - ft * W1;
-      const int c = (ft + 1) * W2 + (fx + 1);
-      const V2 kt = LKt[c], kx = LKx[c], ktd = LKt[c - W2], kxd = LKx[c - 1];
-      const V2 r0 = R2[c], rtp = R2[c + W2], rtm = R2[c - W2], rxp = R2[c + 1], rxm = R2[c - 1];
-      // all nine loads in flight before the arithmetic: one LDS latency per
-      // element (two waves per SIMD hide little)
-      __builtin_amdgcn_sched_barrier(0);
-      // y = m r + e^mu K_t r(t+1) - e^-mu K_t(t-1)* r(t-1) + K_x r(x+1) - K_x(x-1)* r(x-1)
-      V2 y = cscale(r0, m);
-      cfma(y, kt, rtp, epm);
-      cjfma(y, ktd, rtm, -emm);
-      cfma(y, kx, rxp, 1.0f);
-      cjfma(y, kxd, rxm, -1.0f);
-      Y1[e] = y;
+      for (int q = 0; q < NO; q++, c += W2, yo += W1) {
+        const V2 kt = LKt[c], kx = LKx[c], kxd = LKx[c - 1];
+        const V2 rtp = R2[c + W2], rxp = R2[c + 1], rxm = R2[c - 1];
+        // all loads in flight before the arithmetic: one LDS latency per
+        // element (two waves per SIMD hide little)
+        __builtin_amdgcn_sched_barrier(0);
+        // y = m r + e^mu K_t r(t+1) - e^-mu K_t(t-1)* r(t-1) + K_x r(x+1) - K_x(x-1)* r(x-1)
+        V2 y = cscale(r0, m);
+        cfma(y, kt, rtp, epm);
+        cjfma(y, ktd, rtm, -emm);
+        cfma(y, kx, rxp, 1.0f);
+        cjfma(y, kxd, rxm, -1.0f);
+        *yo = y;
+        rtm = r0;
+        r0 = rtp;
+        ktd = kt;
+      }
+      for (int e = tid; e < 2 * W1 + 2 * TT; e += NT) {
+        int ft, fx;
+        if (e < 2 * W1) {  // rows 0 and TT + 1, every column
+          ft = e < W1 ? 0 : TT + 1;
+          fx = e < W1 ? e : e - W1;
+        } else {  // columns 0 and W1 - 1 of rows 1 .. TT
+          const int k = e - 2 * W1;
+          ft = 1 + (k >> 1);
+          fx = (k & 1) ? W1 - 1 : 0;
+        }
+        const int cc = (ft + 1) * W2 + (fx + 1);
+        const V2 kt = LKt[cc], kx = LKx[cc], ktd2 = LKt[cc - W2], kxd = LKx[cc - 1];
+        const V2 r00 = R2[cc], rtp = R2[cc + W2], rtm2 = R2[cc - W2], rxp = R2[cc + 1],
+                 rxm = R2[cc - 1];
+        __builtin_amdgcn_sched_barrier(0);
+        V2 y = cscale(r00, m);
+        cfma(y, kt, rtp, epm);
+        cjfma(y, ktd2, rtm2, -emm);
+        cfma(y, kx, rxp, 1.0f);
+        cjfma(y, kxd, rxm, -1.0f);
+        Y1[ft * W1 + fx] = y;
+      }
     }
     __syncthreads();
     // 3c. w_{j+1} = M^dag y on the region; partials of iteration j+1 (a
-    // thread's sites in FP32, then FP64)
+    // thread's sites in FP32, then FP64).  Going down the wave's rows, y(t-1),
+    // y(t) and K_t(t-1) come from the previous row's registers.  Elements
+    // outside the lattice (a region larger than the lattice) are computed
+    // too, to keep that chain, and their results dropped.
     T gacc32 = 0, dacc32 = 0, racc32 = 0, sacc32 = 0;
+    {
+      int c1 = (wrow + 1) * W1 + (lane + 1);   // Y1 index of (wrow, lane)
+      int c2i = (wrow + 2) * W2 + (lane + 2);  // R2 / link index of (wrow, lane)
+      V2 ytm = Y1[c1 - W1], y0 = Y1[c1], ktd = LKt[c2i - W2];
 #pragma unroll
-    for (int q = 0; q < NO; q++) {
-      if (!((inl >> q) & 1u)) continue;
-      const int lt = wrow + q * (NT / TX), lx = lane;
-      const int c1 = (lt + 1) * W1 + (lx + 1);
-      const int c2i = (lt + 2) * W2 + (lx + 2);
-      const V2 kt = LKt[c2i], kx = LKx[c2i], ktd = LKt[c2i - W2], kxd = LKx[c2i - 1];
-      const V2 y0 = Y1[c1], ytp = Y1[c1 + W1], ytm = Y1[c1 - W1], yxp = Y1[c1 + 1], yxm = Y1[c1 - 1];
-      __builtin_amdgcn_sched_barrier(0);  // one LDS latency per element (as in 3b)
-      // w = m y - e^-mu K_t y(t+1) + e^mu K_t(t-1)* y(t-1) - K_x y(x+1) + K_x(x-1)* y(x-1)
-      V2 v = cscale(y0, m);
-      cfma(v, kt, ytp, -emm);
-      cjfma(v, ktd, ytm, epm);
-      cfma(v, kx, yxp, -1.0f);
-      cjfma(v, kxd, yxm, 1.0f);
-      w[q] = v;
-      if ((pub >> q) & 1u) M.st2(o_w[ns], pub_at(lt, lx), v);
-      gacc32 += r[q].x * r[q].x + r[q].y * r[q].y;
-      dacc32 += r[q].x * v.x + r[q].y * v.y;
-      if (setup && cont) {
-        racc32 += r[q].x * s[q].x + r[q].y * s[q].y;
-        sacc32 += p[q].x * s[q].x + p[q].y * s[q].y;
+      for (int q = 0; q < NO; q++, c1 += W1, c2i += W2) {
+        const int lt = wrow + q, lx = lane;
+        const V2 kt = LKt[c2i], kx = LKx[c2i], kxd = LKx[c2i - 1];
+        const V2 ytp = Y1[c1 + W1], yxp = Y1[c1 + 1], yxm = Y1[c1 - 1];
+        __builtin_amdgcn_sched_barrier(0);  // one LDS latency per element (as in 3b)
+        // w = m y - e^-mu K_t y(t+1) + e^mu K_t(t-1)* y(t-1) - K_x y(x+1) + K_x(x-1)* y(x-1)
+        V2 v = cscale(y0, m);
+        cfma(v, kt, ytp, -emm);
+        cjfma(v, ktd, ytm, epm);
+        cfma(v, kx, yxp, -1.0f);
+        cjfma(v, kxd, yxm, 1.0f);
+        ytm = y0;
+        y0 = ytp;
+        ktd = kt;
+        if (!((inl >> q) & 1u)) continue;
+        w[q] = v;
+        if ((pub >> q) & 1u) M.st2(o_w[ns], pub_at(lt, lx), v);
+        gacc32 += r[q].x * r[q].x + r[q].y * r[q].y;
+        dacc32 += r[q].x * v.x + r[q].y * v.y;
+        if (setup && cont) {
+          racc32 += r[q].x * s[q].x + r[q].y * s[q].y;
+          sacc32 += p[q].x * s[q].x + p[q].y * s[q].y;
+        }
       }
     }
     // 4. the partials of each wave (fixed lane order), stored by its lane 0:
