@@ -216,6 +216,12 @@ int th2d_cg_last_resident(th2d_ctx *ctx, int *on);
  * set by th2d_set_cg_tile, the step kernel solves them.  Replaces no
  * reference function. */
 int th2d_set_cg_small(th2d_ctx *ctx, int on);
+/* Regions of the resident kernel: 64 x 64 sites, or 32 x 64 for batches
+ * whose lattices all cut so and whose twice as many regions still fit the
+ * device in one launch (a single 256^2 chain then runs on 32 CUs instead
+ * of 16).  rows = 0 (default) chooses per batch; 64 or 32 forces the shape
+ * where the batch allows it.  Replaces no reference function. */
+int th2d_set_cg_resident_rows(th2d_ctx *ctx, int rows);
 /* Bounded waits of the resident solver.  Every wait between the regions
  * of a chain gives up after `polls` polls (polls < 0: the default, 2^20,
  * about 1 s) and ends every workgroup of the launch normally; the solve is

@@ -1,5 +1,6 @@
 // Host check of the resident solver's publication layout (res_layout.h),
-// for every even lattice up to NMAX x NMAX: a region publishes exactly the
+// for regions of 64 x 64 and 32 x 64 sites and every even lattice up to
+// NMAX x NMAX: a region publishes exactly the
 // sites within 2 of its clipped edge, into unique slots below 496; every
 // site that an owned site's M^dag M result depends on (L1 distance <= 2)
 // and that the region does not own is published by its owner; every
@@ -14,13 +15,14 @@ int main(int argc, char **argv)
 {
   const int nmax = argc > 1 ? atoi(argv[1]) : 200;
   long bad = 0, checked = 0, dummy = 0;
+  for (int tt : {64, 32})  // regions of 64 x 64 and 32 x 64 sites
   for (int nt = 2; nt <= nmax; nt += 2)
     for (int nx = 2; nx <= nmax; nx += 2) {
-      const int ntt = (nt + 63) / 64, ntx = (nx + 63) / 64;
+      const int ntt = (nt + tt - 1) / tt, ntx = (nx + 63) / 64;
       // uniqueness and range
       for (int rt = 0; rt < ntt; rt++)
         for (int rx = 0; rx < ntx; rx++) {
-          int te = std::min(64, nt - rt * 64), xe = std::min(64, nx - rx * 64);
+          int te = std::min(tt, nt - rt * tt), xe = std::min(64, nx - rx * 64);
           std::set<int> used;
           for (int lt = 0; lt < te; lt++)
             for (int lx = 0; lx < xe; lx++) {
@@ -41,8 +43,8 @@ int main(int argc, char **argv)
                   int a = lt + dt, b = lx + dx;
                   if (a >= 0 && a < te && b >= 0 && b < xe) { need = true; break; }
                 }
-              int t = ((rt * 64 + lt) % nt + nt) % nt, x = ((rx * 64 + lx) % nx + nx) % nx, k;
-              int s = res_site_slot(t, x, nt, nx, k);
+              int t = ((rt * tt + lt) % nt + nt) % nt, x = ((rx * 64 + lx) % nx + nx) % nx, k;
+              int s = res_site_slot(t, x, nt, nx, k, tt);
               checked++;
               if (need && s < 0) { if (bad++ < 10) printf("needed but unpublished %d %d region (%d,%d) (%d,%d)\n", nt, nx, rt, rx, lt, lx); }
               if (!need && s < 0) dummy++;

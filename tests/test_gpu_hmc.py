@@ -336,6 +336,49 @@ def test_cg_resident_matches_step_kernel(hmc, oracle, mu1):
         assert abs(it1 - it0) <= max(3, it0 // 10)
 
 
+@pytest.mark.parametrize("case", ["one_256", "mixed"])
+def test_cg_resident_half_rows(hmc, oracle, case):
+    """Resident regions of 32 x 64 sites (th2d_set_cg_resident_rows): a
+    single 256^2 chain (32 regions on one XCD), and a batch of 128^2, 64 x
+    192, 32 x 64 and 64^2 with a chemical potential on one chain (the
+    kernel instance with the e^{+-mu} factors).  With 32 rows, with 64 and
+    with the automatic choice (32 here: the doubled regions still fit)
+    every solve reaches the true residual against the oracle, and the
+    iteration counts of the two shapes agree within 10%."""
+    import ctypes as C
+    from thirring2d_amd import _native as N
+    dims = [(256, 256)] if case == "one_256" else [(128, 128), (64, 192), (32, 64), (64, 64)]
+    setups = [_setup(hmc, oracle, nt, nx, 700 + k, m=0.05 + 0.02 * k,
+                     mu=0.2 if (case == "mixed" and k == 2) else 0.0)
+              for k, (nt, nx) in enumerate(dims)]
+    batch = hmc.ChainBatch([s[0] for s in setups])
+    ctx = setups[0][0].ctx
+    lib = N.lib()
+    runs = {}
+    tt, tx = C.c_int(0), C.c_int(0)
+    lib.th2d_set_cg_reliable(ctx.ptr, 0)
+    try:
+        for rows in (64, 32, 0):
+            assert lib.th2d_set_cg_resident_rows(ctx.ptr, rows) == 0
+            xs, infos = batch.fmdm_invert_cg([s[3] for s in setups],
+                                             hmc.cg_params(tol_rel=1e-10, precision=N.PREC_MIXED))
+            assert _last_resident(ctx) == 1
+            assert lib.th2d_cg_last_tile(ctx.ptr, C.byref(tt), C.byref(tx)) == 0
+            assert (tt.value, tx.value) == ((64, 64) if rows == 64 else (32, 64)), rows
+            runs[rows] = (xs, infos)
+    finally:
+        lib.th2d_set_cg_resident_rows(ctx.ptr, 0)
+    assert lib.th2d_set_cg_resident_rows(ctx.ptr, 48) == N.TH2D_EARG
+    for k, (lat, L, A, b, _) in enumerate(setups):
+        for rows in (64, 32, 0):
+            x, inf = runs[rows][0][k], runs[rows][1][k]
+            assert inf.status == 0
+            r = b - oracle.fm_dag_mul(L, A, oracle.fm_mul(L, A, x))
+            assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10, (dims[k], rows)
+        it64, it32 = runs[64][1][k].iterations, runs[32][1][k].iterations
+        assert abs(it64 - it32) <= max(3, it64 // 10), (dims[k], it64, it32)
+
+
 @pytest.mark.parametrize("m", [0.2, 0.05])
 def test_cg_reliable_updates(hmc, oracle, m):
     """Mixed solves with reliable updates (the FP32 Krylov sequence continues

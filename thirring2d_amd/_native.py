@@ -127,6 +127,7 @@ SIGNATURES = {
     "th2d_set_cg_graph": (I, [P, I]),
     "th2d_set_cg_reliable": (I, [P, I]),
     "th2d_cg_last_resident": (I, [P, C.POINTER(I)]),
+    "th2d_set_cg_resident_rows": (I, [P, I]),
     "th2d_set_cg_spin_limit": (I, [P, L]),
     "th2d_set_cg_small": (I, [P, I]),
     "th2d_cg_sync_fallbacks": (I, [P, C.POINTER(L)]),

@@ -455,10 +455,13 @@ __global__ void __launch_bounds__(S::NT) k_cg_step(CgBatch<T> a, int slot)
 // of a chain take bitwise the same stop decision and leave together; every
 // wait is bounded (a wait that times out sets the error flag and ends its
 // workgroup).
+// Regions of 64 x 64 sites, and of 32 x 64 for batches small enough that
+// halving the regions still leaves one region per CU (one chain of 256^2:
+// 32 regions instead of 16).
 using RShape = Shape<64, 64, 512>;
-static_assert(RShape::TT == RES_T && RShape::TX == RES_T, "regions of the publication layout");
-constexpr int RES_NR = 4 * RShape::W2 + 4 * RShape::TT;  // halo elements of a region
-constexpr int RES_NQ = (RES_NR + RShape::NT - 1) / RShape::NT;
+using RShapeH = Shape<32, 64, 512>;
+static_assert(RShape::TT == RES_T && RShape::TX == RES_T && RShapeH::TX == RES_T,
+              "regions of the publication layout");
 constexpr unsigned RES_SPIN_LIMIT = 1u << 20;            // polls of one wait (about 1 s)
 constexpr unsigned kNoLoad = 0xFFFFFFFFu;                 // halo element no result needs
 
@@ -606,9 +609,10 @@ __device__ bool agent_sync(unsigned *bar, unsigned target, int *err, double *fla
 
 // Position (row, column) in the (TT+4) x (TX+4) halo frame of halo element
 // q: two rows above, two below, then two columns left and right.
+template <typename S>
 __device__ __forceinline__ void res_ring_pos(int q, int &et, int &ex)
 {
-  constexpr int W2 = RShape::W2, TT = RShape::TT, TX = RShape::TX;
+  constexpr int W2 = S::W2, TT = S::TT, TX = S::TX;
   if (q < 2 * W2) {
     et = q / W2;
     ex = q - et * W2;
@@ -681,17 +685,22 @@ __device__ __forceinline__ int xcc_id()
   return (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u);
 }
 
-// MU0: every chain of the batch has mu = 0 (the e^{+-mu} factors are 1)
-template <typename T, bool MU0>
-__global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
+// MU0: every chain of the batch has mu = 0 (the e^{+-mu} factors are 1);
+// S: the region shape (RShape or RShapeH)
+template <typename T, bool MU0, typename S>
+__global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
 {
-  using S = RShape;
   using V2 = c2<T>;
   static_assert(sizeof(V2) == 8, "the resident solver is FP32");
   constexpr int TT = S::TT, TX = S::TX, NT = S::NT, W2 = S::W2, W1 = S::W1;
   constexpr int E2 = S::E2, E1 = S::E1, NO = S::NO;
   constexpr int LDS_D = (3 * E2 + E1) * (int)sizeof(V2) / 8;
-  __shared__ double rlds[LDS_D + 40];
+  constexpr int NR = 4 * W2 + 4 * TT;   // halo ring elements of a region
+  constexpr int NQ = (NR + NT - 1) / NT;
+  // more than half of a CU's LDS, so that one workgroup runs per CU (the
+  // 32-row regions would otherwise share CUs and gain nothing)
+  constexpr int LDS_W = LDS_D + 40 > 82 * 1024 / 8 ? LDS_D + 40 : 82 * 1024 / 8;
+  __shared__ double rlds[LDS_W];
   V2 *R2 = (V2 *)rlds, *Y1 = R2 + E2, *LKt = Y1 + E1, *LKx = LKt + E2;
   double *red = rlds + LDS_D;
   const int tid = threadIdx.x;
@@ -732,7 +741,8 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
     unsigned off = kNoLoad;
     if (lt < tte + 2 && lx < txe + 2) {
       int k;
-      const int sl = res_site_slot(wrapn(tl.t0 + lt, G.nt), wrapn(tl.x0 + lx, G.nx), G.nt, G.nx, k);
+      const int sl =
+          res_site_slot(wrapn(tl.t0 + lt, G.nt), wrapn(tl.x0 + lx, G.nx), G.nt, G.nx, k, TT);
       if (sl >= 0) off = (unsigned)(ch.tbeg + k) * PB + (unsigned)sl * (unsigned)sizeof(V2);
     }
     return off;
@@ -780,15 +790,15 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       s[q] = cscale(*at((const V2 *)ch.cs, oi), cf);
     }
   }
-  unsigned ri[RES_NQ];  // publication offsets of the halo elements (kNoLoad: none)
-  int re[RES_NQ];       // their places in R2
+  unsigned ri[NQ];  // publication offsets of the halo elements (kNoLoad: none)
+  int re[NQ];       // their places in R2
 #pragma unroll
-  for (int h = 0; h < RES_NQ; h++) {
+  for (int h = 0; h < NQ; h++) {
     ri[h] = kNoLoad;
     re[h] = 0;
-    if (tid + h * NT < RES_NR) {
+    if (tid + h * NT < NR) {
       int et, ex;
-      res_ring_pos(tid + h * NT, et, ex);
+      res_ring_pos<S>(tid + h * NT, et, ex);
       ri[h] = src_at(et - 2, ex - 2);
       re[h] = et * W2 + ex;
     }
@@ -818,7 +828,7 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
   // iteration -1 (set-up): w_0 = A b
   bool setup = true;
   T ta = 0, tb = 0;
-  V2 hr[RES_NQ], hw[RES_NQ], hs[RES_NQ];  // halo values of the next iteration
+  V2 hr[NQ], hw[NQ], hs[NQ];  // halo values of the next iteration
   for (int j = -1;; j++) {
     const int ns = setup ? 0 : slot ^ 1;  // parity written by this pass
     if (!setup) {
@@ -861,8 +871,8 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
         }
       }
 #pragma unroll
-      for (int h = 0; h < RES_NQ; h++) {
-        if (tid + h * NT >= RES_NR) continue;
+      for (int h = 0; h < NQ; h++) {
+        if (tid + h * NT >= NR) continue;
         const V2 sn = cadd(hw[h], cscale(hs[h], tb));
         R2[re[h]] = csub(hr[h], cscale(sn, ta));
       }
@@ -1009,8 +1019,8 @@ __global__ void __launch_bounds__(RShape::NT) k_cg_resident(CgBatch<T> a, ResCtl
       }
     }
 #pragma unroll
-    for (int h = 0; h < RES_NQ; h++) {
-      if (tid + h * NT >= RES_NR) continue;
+    for (int h = 0; h < NQ; h++) {
+      if (tid + h * NT >= NR) continue;
       hr[h] = hw[h] = hs[h] = zero;
       if (ri[h] != kNoLoad) {
         hr[h] = M.ld2(o_r[slot], ri[h]);
@@ -1912,12 +1922,15 @@ __global__ void k_zero_u32(unsigned *p, int n)
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
-// Regions of the resident solver for a batch.
-int res_regions(int nch, const Geo *G)
+// Regions of a lattice, and of a batch, for regions of tt rows.
+inline long res_chain_regions(const Geo &g, int tt)
+{
+  return (long)((g.nt + tt - 1) / tt) * ((g.nx + RES_T - 1) / RES_T);
+}
+int res_regions(int nch, const Geo *G, int tt = RES_T)
 {
   long n = 0;
-  for (int c = 0; c < nch; c++)
-    n += (long)((G[c].nt + RShape::TT - 1) / RShape::TT) * ((G[c].nx + RShape::TX - 1) / RShape::TX);
+  for (int c = 0; c < nch; c++) n += res_chain_regions(G[c], tt);
   return n > (1L << 30) ? (1 << 30) : (int)n;
 }
 
@@ -1927,21 +1940,24 @@ int res_regions(int nch, const Geo *G)
 // XCDs were full (wrong halo values in those chains only), while a CPU
 // emulation of the kernel with random schedules solved the same batches
 // exactly; until the cause is known the step kernel solves such batches.
-bool res_eligible(const Geo &g)
+bool res_eligible(const Geo &g, int tt = RES_T)
 {
-  const bool one = g.nt <= RES_T && g.nx <= RES_T;
-  return one || (g.nt % RES_T == 0 && g.nx % RES_T == 0);
+  const bool one = g.nt <= tt && g.nx <= RES_T;
+  return one || (g.nt % tt == 0 && g.nx % RES_T == 0);
 }
 
-// Resident workgroups the device holds (cooperative launch bound).
+// Resident workgroups the device holds (cooperative launch bound); the
+// same for both region shapes (one workgroup per CU).
 int res_capacity(th2d_ctx *ctx)
 {
   if (ctx->res_capacity < 0) {
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)k_cg_resident<float, true>,
-                                                     RShape::NT, 0) != hipSuccess)
-      occ = 0;
-    ctx->res_capacity = occ * ctx->num_cu;
+    int occ = 0, occh = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &occ, (const void *)k_cg_resident<float, true, RShape>, RShape::NT, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &occh, (const void *)k_cg_resident<float, true, RShapeH>, RShapeH::NT, 0) != hipSuccess)
+      occ = occh = 0;
+    ctx->res_capacity = std::min(occ, occh) * ctx->num_cu;
   }
   return ctx->res_capacity;
 }
@@ -1961,12 +1977,12 @@ struct ResGroup {
   size_t tab;      // offset of its table in wg_tile
   long sites;      // lattice sites it solves
 };
-bool place_regions(int nch, const Geo *G, int per_xcd, std::vector<int> &wg_tile,
+bool place_regions(int nch, const Geo *G, int per_xcd, int tt, std::vector<int> &wg_tile,
                    std::vector<ResGroup> &groups)
 {
   std::vector<int> ntc(nch), tbeg(nch + 1, 0), order(nch);
   for (int c = 0; c < nch; c++) {
-    ntc[c] = ((G[c].nt + RShape::TT - 1) / RShape::TT) * ((G[c].nx + RShape::TX - 1) / RShape::TX);
+    ntc[c] = (int)res_chain_regions(G[c], tt);
     if (ntc[c] > per_xcd) return false;
     tbeg[c + 1] = tbeg[c] + ntc[c];
     order[c] = c;
@@ -2017,18 +2033,19 @@ bool place_regions(int nch, const Geo *G, int per_xcd, std::vector<int> &wg_tile
 // about 8.5 us (measured at 16 lattices of 256^2, set by the per-iteration
 // synchronisation), one of the step kernel about 10 us plus 120 B per site
 // at 4.5 TB/s (measured).  Many small lattices favour the step kernel.
-int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
-                   const c2<float> *const *rhs, const c2<float> *const *K, const double *stop_rr,
-                   const int *max_iter, c2<float> **xout, InnerResult *res, const Carry *carry)
+template <typename S>
+int resident_solve_s(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
+                     const c2<float> *const *rhs, const c2<float> *const *K, const double *stop_rr,
+                     const int *max_iter, c2<float> **xout, InnerResult *res, const Carry *carry)
 {
-  const int nreg = res_regions(nch, G);
+  const int nreg = res_regions(nch, G, S::TT);
   const int cap = res_capacity(ctx);
   if (nreg < 1 || cap < kNumXcd) return TH2D_ELAUNCH;
   for (int c = 0; c < nch; c++)
-    if (!res_eligible(G[c])) return TH2D_ELAUNCH;
+    if (!res_eligible(G[c], S::TT)) return TH2D_ELAUNCH;
   std::vector<int> wg_tile;
   std::vector<ResGroup> groups;
-  if (!place_regions(nch, G, cap / kNumXcd, wg_tile, groups)) return TH2D_ELAUNCH;
+  if (!place_regions(nch, G, cap / kNumXcd, S::TT, wg_tile, groups)) return TH2D_ELAUNCH;
   long sites = 0;
   for (const ResGroup &g : groups) {
     if (g.grid < 1 || g.grid > cap) return TH2D_ELAUNCH;
@@ -2041,19 +2058,18 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   // partials and the publication planes
   int nparts = 0;
   for (int c = 0; c < nch; c++)
-    nparts += ((int)(((G[c].nt + RShape::TT - 1) / RShape::TT) *
-                     ((G[c].nx + RShape::TX - 1) / RShape::TX)) + 7) & ~7;
+    nparts += ((int)res_chain_regions(G[c], S::TT) + 7) & ~7;
   const size_t ntab = (wg_tile.size() + 31) & ~(size_t)31;
   const size_t nzero = 64 * (size_t)nch + 32;  // counters and error flags, zeroed
   const size_t nxcc = ((size_t)nreg + 31) & ~(size_t)31;
   const size_t pub_bytes = 6 * (size_t)nreg * RES_PUB * sizeof(c2<float>);  // 2 x (r, s, w)
-  constexpr size_t NW = RShape::NT / 64;  // partial slots of a region: one per wave
+  constexpr size_t NW = S::NT / 64;  // partial slots of a region: one per wave
   const size_t extra_bytes = 4 * (ntab + nzero + nxcc) + 64 * NW * (size_t)nparts + pub_bytes;
   CgBatch<float> a;
   std::vector<CgChain> ch;
   void *extra = nullptr;
-  TH2D_TRY(build_batch<float>(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, RShape::TT,
-                              RShape::TX, extra_bytes, a, ch, &extra, carry, wg_tile.data(),
+  TH2D_TRY(build_batch<float>(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, S::TT, S::TX,
+                              extra_bytes, a, ch, &extra, carry, wg_tile.data(),
                               sizeof(int) * wg_tile.size()));
   if (a.ntiles != nreg) return TH2D_EARG;
   ResCtl ctl;
@@ -2081,8 +2097,9 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   for (const ResGroup &g : groups) {  // one launch per group (each chain in one group)
     ctl.wg_tile = tab0 + g.tab;
     void *args[] = {(void *)&a, (void *)&ctl};
-    TH2D_HIP(hipLaunchCooperativeKernel(mu0 ? k_cg_resident<float, true> : k_cg_resident<float, false>,
-                                        dim3(g.grid), dim3(RShape::NT), args, 0, ctx->stream));
+    TH2D_HIP(hipLaunchCooperativeKernel(
+        mu0 ? k_cg_resident<float, true, S> : k_cg_resident<float, false, S>, dim3(g.grid),
+        dim3(S::NT), args, 0, ctx->stream));
   }
   // results and the two flags into one pinned block, then one wait
   const size_t rb = sizeof(CgResult) * nch;
@@ -2097,20 +2114,41 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   memcpy(flags, (char *)ctx->redh.ptr + rb, sizeof(flags));
   if (flags[1]) return TH2D_ELAUNCH;  // placement not as dispatched: the step kernel solves
   if (flags[0]) return TH2D_ESYNC;
-  ctx->cg_last_tile[0] = RShape::TT;
-  ctx->cg_last_tile[1] = RShape::TX;
+  ctx->cg_last_tile[0] = S::TT;
+  ctx->cg_last_tile[1] = S::TX;
   ctx->cg_last_resident = 1;
   TH2D_TRY(finish_solve<float>(ctx, nch, G, a, ch, h, (long)groups.size(), xout, res));
   for (int c = 0; c < nch; c++) res[c].resident = 1;
   return TH2D_OK;
 }
 
+// The resident solve with regions of 32 rows when every chain of the batch
+// can be cut so (sides multiples of 32 x 64, or one region) and twice the
+// regions still fit the device in one launch; of 64 rows otherwise.
+// th2d_set_cg_resident_rows forces either.
+int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
+                   const c2<float> *const *rhs, const c2<float> *const *K, const double *stop_rr,
+                   const int *max_iter, c2<float> **xout, InnerResult *res, const Carry *carry)
+{
+  bool half = ctx->res_rows == RShapeH::TT;
+  if (ctx->res_rows == 0) {
+    const int cap = res_capacity(ctx);
+    half = res_regions(nch, G, RShapeH::TT) <= cap;
+    for (int c = 0; c < nch && half; c++)
+      half = res_eligible(G[c], RShapeH::TT) && res_eligible(G[c], RShape::TT) &&
+             res_chain_regions(G[c], RShapeH::TT) <= cap / kNumXcd;
+  }
+  if (half)
+    return resident_solve_s<RShapeH>(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, xout, res,
+                                     carry);
+  return resident_solve_s<RShape>(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, xout, res, carry);
+}
+
 // Whether the resident solver takes a lattice: eligible, and no more
 // regions than one XCD holds.
 bool res_takes(const Geo &g, int cap)
 {
-  const long r = (long)((g.nt + RShape::TT - 1) / RShape::TT) * ((g.nx + RShape::TX - 1) / RShape::TX);
-  return res_eligible(g) && r <= cap / kNumXcd;
+  return res_eligible(g) && res_chain_regions(g, RShape::TT) <= cap / kNumXcd;
 }
 
 template <typename T>
@@ -2321,6 +2359,13 @@ extern "C" int th2d_set_cg_small(th2d_ctx *ctx, int on)
 {
   if (!ctx) return TH2D_EARG;
   ctx->cg_small = on ? 1 : 0;
+  return TH2D_OK;
+}
+
+extern "C" int th2d_set_cg_resident_rows(th2d_ctx *ctx, int rows)
+{
+  if (!ctx || (rows != 0 && rows != RShape::TT && rows != RShapeH::TT)) return TH2D_EARG;
+  ctx->res_rows = rows;
   return TH2D_OK;
 }
 

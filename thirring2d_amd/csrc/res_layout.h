@@ -1,8 +1,9 @@
 // Boundary exchange layout of the resident solver (k_cg_resident).
 //
-// A lattice is cut into RES_T x RES_T regions, numbered t-major within a
-// chain (region k covers rows 64 (k / ntx) .., columns 64 (k % ntx) ..;
-// the last row and column of regions are clipped to the lattice).  Each
+// A lattice is cut into regions of RES_T (or RES_T / 2) rows and RES_T
+// columns, numbered t-major within a chain (region k covers rows
+// tt (k / ntx) .., columns 64 (k % ntx) ..; the last row and column of
+// regions are clipped to the lattice).  Each
 // region publishes the sites within 2 sites of its clipped edge into a
 // block of RES_PUB elements of its own: the top two rows, the bottom two
 // rows, then for every other row its two left and its two right columns,
@@ -34,12 +35,13 @@ RES_HD inline int res_pub_slot(int lt, int lx, int te, int xe)
 }
 
 // Region (index within the chain, into k) and publication slot of lattice
-// site (t, x), 0 <= t < nt, 0 <= x < nx; -1 when the site is not published.
-RES_HD inline int res_site_slot(int t, int x, int nt, int nx, int &k)
+// site (t, x), 0 <= t < nt, 0 <= x < nx, for regions of tt rows (64 or 32)
+// and RES_T columns; -1 when the site is not published.
+RES_HD inline int res_site_slot(int t, int x, int nt, int nx, int &k, int tt = RES_T)
 {
-  const int rt = t / RES_T, rx = x / RES_T;
+  const int rt = t / tt, rx = x / RES_T;
   k = rt * ((nx + RES_T - 1) / RES_T) + rx;
-  const int te = nt - rt * RES_T < RES_T ? nt - rt * RES_T : RES_T;
+  const int te = nt - rt * tt < tt ? nt - rt * tt : tt;
   const int xe = nx - rx * RES_T < RES_T ? nx - rx * RES_T : RES_T;
-  return res_pub_slot(t - rt * RES_T, x - rx * RES_T, te, xe);
+  return res_pub_slot(t - rt * tt, x - rx * RES_T, te, xe);
 }

@@ -74,6 +74,7 @@ struct th2d_ctx {
   int hmc_graph = 1;              // small-lattice FP64 MD steps as graph replays
   int cg_small = 1;               // FP64 solves of lattices up to 2048 sites in one launch
   long res_spin_limit = -1;       // polls of one resident-kernel wait (-1: the default)
+  int res_rows = 0;               // resident regions of 64 or 32 rows (0: chosen per batch)
   long cg_sync_fallbacks = 0;     // resident solves that timed out and were re-solved
   std::map<unsigned long long, int> cg_tuned[2];  // signature -> shape (FP32, FP64)
   // step-kernel launches captured as graphs (cg.hip), keyed by the launch
