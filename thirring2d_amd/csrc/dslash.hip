@@ -181,7 +181,6 @@ template <typename T, int DAG>
 static void launch_t(th2d_ctx *c, const Geo &G, void *out, const void *in, const void *K, int nrhs)
 {
   using V2 = c2<T>;
-  int V = 2 * G.vh;
   if (dslash_variant() == 1) {
     constexpr int R = 4, CC = 64;
     int tiles = ((G.nt + R - 1) / R) * ((G.nx + CC - 1) / CC);
