@@ -72,7 +72,7 @@ def parse():
                     help="trajectories per chain of the 256-chain uneven batch (0 = skip)")
     ap.add_argument("--c5-proposals", type=int, default=150,
                     help="bag proposals per matrix in the dense leg (0 = skip the leg)")
-    ap.add_argument("--c1-steps", type=int, default=40,
+    ap.add_argument("--c1-steps", type=int, default=100,
                     help="16^2 FP64 trajectories (0 = skip)")
     ap.add_argument("--only-legs", default="", metavar="LIST",
                     help="run only these legs, comma separated, and exit: c1, c4, "
@@ -200,31 +200,61 @@ def mixed_vs_fp64(H, N, lats, nsteps, tau=1e-10):
             "bound": "4 tau |xi|^2 sqrt(m^2+4)/m per chain, tau = 1e-10"}
 
 
-def c4_leg(H, N, steps, nsteps=64):
-    """256 chains, L uniform on {32,48,64,96,128} (seed 4004), g ~ U[0.2,1],
-    m = 0.1, mixed CG to 1e-10, autotuned tiles; every chain starts as
-    hmc.c:927-929 does (A = 0, 100 pure-gauge sweeps) and runs `steps`
-    trajectories of nsteps x dt = 1/nsteps."""
+def c4_batch(H, start):
+    """The C4 chains: 256 lattices, L uniform on {32,48,64,96,128} (seed
+    4004), g ~ U[0.2,1], m = 0.1; A ~ N(0,1) per chain (start "gauss", the
+    configuration as specified) or A = 0 plus 100 pure-gauge sweeps as
+    hmc.c:927-929 starts (start "thermal")."""
     rng = np.random.default_rng(4004)
     Ls = rng.choice([32, 48, 64, 96, 128], size=256)
     gs = rng.uniform(0.2, 1.0, size=256)
     lats = []
     for c, (Lc, gc) in enumerate(zip(Ls, gs)):
         lat = H.StaggeredLattice(int(Lc), int(Lc), m=0.1, g=float(gc))
-        lat.thermalize_pure_gauge(100, seed=40040 + c)
+        if start == "gauss":
+            lat.random_links(40040 + c)
+        else:
+            lat.thermalize_pure_gauge(100, seed=40040 + c)
         lats.append(lat)
+    return Ls, gs, lats
+
+
+def c4_run(H, N, lats, gs, Ls, steps, nsteps):
     dt, res, kms, kn, kby = run_batch(H, N, lats, list(gs), N.PREC_MIXED, steps, 1,
                                       nsteps=nsteps)
-    sites = int(np.sum(Ls.astype(np.int64) ** 2))
     dH = np.array([r[0] for r in res])
-    return {"value": 256 * steps / dt, "unit": "traj/s", "chains": 256, "sites": sites,
-            "trajectories": 256 * steps, "nsteps": nsteps, "dt": 1.0 / nsteps,
-            "sizes": "L in {32,48,64,96,128}, seed 4004",
+    return {"value": 256 * steps / dt, "unit": "traj/s", "trajectories": 256 * steps,
+            "nsteps": nsteps, "dt": 1.0 / nsteps,
             "acceptance": float(np.mean([r[1] for r in res])),
             "exp_minus_dH_mean": float(np.mean(np.exp(-np.minimum(dH, 50.0)))),
             "dH_median": float(np.median(dH)),
             "max_true_rel_resid": max(r[4] for r in res),
             "cg_kernel_GBs": kby / (kms * 1e-3) / 1e9 if kms > 0 else 0.0}
+
+
+def c4_leg(H, N, steps, nsteps=64):
+    """C4: 256 chains, mixed CG to 1e-10, autotuned tiles, `steps`
+    trajectories each.
+    - as specified: A ~ N(0,1), 10 x dt = 0.1 (hmc.c:708-709; beyond the
+      leapfrog stability limit for most chains, so nearly every trajectory
+      is rejected and the fields stay put);
+    - stable: thermalized starts, nsteps x dt = 1/nsteps;
+    - one trajectory of every stable chain with mixed and with FP64 solves
+      from identical fields and noise: per-chain |dH_mixed - dH_fp64|
+      against 4 tau |xi|^2 sqrt(m^2+4)/m."""
+    Ls, gs, lats = c4_batch(H, "gauss")
+    spec = c4_run(H, N, lats, gs, Ls, steps, 10)
+    del lats
+    Ls, gs, lats = c4_batch(H, "thermal")
+    stable = c4_run(H, N, lats, gs, Ls, steps, nsteps)
+    cmp = mixed_vs_fp64(H, N, lats, nsteps)
+    out = {"value": stable["value"], "unit": "traj/s", "chains": 256,
+           "sites": int(np.sum(Ls.astype(np.int64) ** 2)),
+           "sizes": "L in {32,48,64,96,128}, seed 4004",
+           "as_specified_A_gauss_10x0.1": spec, "stable_thermalized": stable,
+           "mixed_vs_fp64_dH": {k: cmp[k] for k in ("chains", "max_abs_dH_diff",
+                                                     "max_diff_over_bound", "bound")}}
+    return out
 
 
 def c1_leg(H, N, steps):
@@ -437,7 +467,7 @@ def only_legs(names):
     out = {}
     for name in names:
         if name == "c1":
-            out["c1_16x16_fp64"] = c1_leg(H, N, 40)
+            out["c1_16x16_fp64"] = c1_leg(H, N, 100)
         elif name == "c4":
             N.lib().th2d_set_cg_autotune(H.Context.get().ptr, 1)
             out["c4_256_chains_uneven"] = c4_leg(H, N, 50)
