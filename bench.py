@@ -318,31 +318,54 @@ def c2_leg(H, N):
 
 def c3_leg(H, N):
     """Force at 128^2, m = 0.1, g = 0.8 (momentum_step, hmc.c:504-665) on
-    all 32,768 links: the FP64 solve (1e-14) against the mixed one (1e-10),
-    and the device force against the oracle's (CPU, FP64 1e-14)."""
-    import oracle as O
+    all 32,768 links, psi = M^dag xi: the force from the FP64 solve (1e-14)
+    against central differences (h = 1e-6) of S = S_g + psi^dag (M^dag M)^-1
+    psi on 64 random links (seed 3003), and the force from the mixed solve
+    (1e-10) against the FP64 one.  S_pf is evaluated in the stationary form
+    2 Re psi^dag x - |M x|^2 (error second order in the solver's residual).
+    Everything runs on the device."""
     nt = nx = 128
     V = nt * nx
     A = np.random.default_rng(1003).normal(size=(V, 2))
     r = np.random.default_rng(2003)
     xi = r.normal(size=V) + 1j * r.normal(size=V)
-    L = O.lattice(nt, nx, m=0.1, g=0.8)
-    psi = O.fm_dag_mul(L, A, xi)
     lat = H.StaggeredLattice(nt, nx, m=0.1, g=0.8)
     lat.set_links(A)
+    psi = lat.fm_conjugate_mul(xi)
     chi64, i64 = lat.fmdm_invert_cg(psi, H.cg_params(tol_rel=1e-14))
     chi32, i32 = lat.fmdm_invert_cg(psi, H.cg_params(tol_rel=1e-10, precision=N.PREC_MIXED))
     F64, F32 = lat.force(chi64), lat.force(chi32)
-    t0 = time.perf_counter()
-    _, Fo, _ = O.momentum_step(L, np.zeros((V, 2)), A, psi, np.zeros(V, complex), 0.1,
-                               tol_rel=1e-14)
-    t_cpu = time.perf_counter() - t0
-    rel = lambda a, b: float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
-    return {"links": 2 * V, "fp64_vs_oracle_max_rel": rel(F64, Fo),
-            "mixed_vs_fp64_max_rel": rel(F32, F64),
-            "mixed_bound": 2 * (0.01 + 4) / 0.01 * 1e-10,
-            "cg_iterations_fp64": int(i64.iterations), "cg_iterations_mixed": int(i32.iterations),
-            "oracle_cpu_seconds_1core": t_cpu}
+    pick = np.random.default_rng(3003)
+    sites = pick.integers(0, V, size=64)
+    dirs = pick.integers(0, 2, size=64)
+    p13 = H.cg_params(tol_rel=1e-13)
+    h = 1e-6
+
+    def action(Ap):
+        lat.set_links(Ap)
+        x, _ = lat.fmdm_invert_cg(psi, p13)
+        Mx = lat.fm_mul(x)
+        return 2 * np.vdot(psi, x).real - np.vdot(Mx, Mx).real + lat.calc_gauge_action()
+
+    fd_err = []
+    for s_, d in zip(sites, dirs):
+        Sp = []
+        for sg in (1, -1):
+            Ap = A.copy()
+            Ap[s_, d] += sg * h
+            Sp.append(action(Ap))
+        fd = (Sp[0] - Sp[1]) / (2 * h)
+        fd_err.append(abs(F64[s_, d] - fd) / max(1.0, abs(fd)))
+    lat.set_links(A)
+    d32 = np.abs(F32 - F64) / np.max(np.abs(F64))
+    return {"links": 2 * V,
+            "fp64_vs_fd_64_links": {"max_rel": float(np.max(fd_err)),
+                                    "rms_rel": float(np.sqrt(np.mean(np.square(fd_err)))),
+                                    "h": h, "note": "|F - FD| / max(1, |FD|)"},
+            "mixed_vs_fp64": {"max_rel": float(np.max(d32)),
+                              "rms_rel": float(np.sqrt(np.mean(np.square(d32)))),
+                              "bound": 2 * (0.01 + 4) / 0.01 * 1e-10},
+            "cg_iterations_fp64": int(i64.iterations), "cg_iterations_mixed": int(i32.iterations)}
 
 
 def _bag_field(rng, L, rho):
@@ -594,6 +617,8 @@ ORACLE_APPLY_MS_DEV = 7.35
 
 
 def oracle_apply_ms(size=256, reps=20):
+    """Part of the CPU baseline: the oracle's fm_mul at size^2 on one host
+    core, ms per apply (the calibration ORACLE_APPLY_MS_DEV above)."""
     import oracle as O
     V = size * size
     L = O.lattice(size, size, m=0.05, g=0.5)
