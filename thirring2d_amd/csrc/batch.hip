@@ -27,13 +27,19 @@ __global__ void __launch_bounds__(BT) k_bop(const BOp *__restrict__ ops, int nch
   const int V = 2 * o.G.vh;
   const int i = (int)(blockIdx.x - o.blk0) * BT + threadIdx.x;
   const bool in = i < V;
-  if (OP == BOP_DOT || OP == BOP_COSGAP) {
+  if (OP == BOP_DOT || OP == BOP_COSGAP || OP == BOP_RESID) {
     __shared__ double sh[8];
     double acc = 0.0;
     if (in) {
       if (OP == BOP_DOT) {
         const double2 a = ((const double2 *)o.d0)[i], b = ((const double2 *)o.d1)[i];
         acc = a.x * b.x + a.y * b.y;
+      } else if (OP == BOP_RESID) {
+        const double2 b = ((const double2 *)o.d1)[i];
+        const double2 m = dslash_site(o.G, (const double2 *)o.d3, (const double2 *)o.d2, i, 1);
+        const double2 r = make_double2(b.x - m.x, b.y - m.y);
+        ((double2 *)o.d0)[i] = r;
+        acc = r.x * r.x + r.y * r.y;
       } else {
         acc = (1.0 - cos(o.d0[i])) + (1.0 - cos(o.d0[V + i]));
       }
@@ -128,7 +134,8 @@ int bop_launch(th2d_ctx *c, int op, const BOp *dev, int nch, long nblk)
 {
   if (nch < 1 || nblk < 1) return TH2D_OK;
   double *part = nullptr;
-  if (op == BOP_DOT || op == BOP_COSGAP) {
+  const bool red = op == BOP_DOT || op == BOP_COSGAP || op == BOP_RESID;
+  if (red) {
     TH2D_TRY(th2d_buf_reserve(c->bpart, sizeof(double) * (size_t)nblk));
     part = (double *)c->bpart.ptr;
   }
@@ -145,9 +152,10 @@ int bop_launch(th2d_ctx *c, int op, const BOp *dev, int nch, long nblk)
   case BOP_SCALE: launch<BOP_SCALE>(c, dev, nch, nblk, part); break;
   case BOP_DOT: launch<BOP_DOT>(c, dev, nch, nblk, part); break;
   case BOP_COSGAP: launch<BOP_COSGAP>(c, dev, nch, nblk, part); break;
+  case BOP_RESID: launch<BOP_RESID>(c, dev, nch, nblk, part); break;
   default: return TH2D_EARG;
   }
-  if (op == BOP_DOT || op == BOP_COSGAP)
+  if (red)
     hipLaunchKernelGGL(k_bop_finish, dim3(nch), dim3(BT), 0, c->stream, dev, nch, nblk, part);
   TH2D_HIP(hipGetLastError());
   return TH2D_OK;

@@ -2525,8 +2525,8 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
                                 (const BOp *)ctx->btab[0].ptr, na, nblk_in));
     }
     // inner solve, then x += |r| y for every chain and the true residual
-    // r = b - M^dag M x in FP64: five batched launches
-    std::vector<BOp> tx(na), tm(na), tmd(na), tsub(na), tdot(na);
+    // r = b - M^dag M x with |r|^2 in FP64: three batched launches
+    std::vector<BOp> tx(na), tm(na), tres(na);
     std::vector<char> was_cont(na, 0);
     if (mixed) {
       std::vector<const float2 *> rh(na), kk(na);
@@ -2590,31 +2590,23 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
       tm[k].d0 = tmp;
       tm[k].d1 = x[c];
       tm[k].d2 = (double *)K[c];
-      tmd[k] = tm[k];  // r = M^dag tmp
-      tmd[k].d0 = r;
-      tmd[k].d1 = tmp;
-      tmd[k].s1 = 1.0;
-      tsub[k] = op_of(c);  // r = b - r
-      tsub[k].d0 = r;
-      tsub[k].d1 = (double *)b[c];
-      tsub[k].d2 = r;
-      tdot[k] = op_of(c);  // |r|^2
-      tdot[k].d0 = r;
-      tdot[k].d1 = r;
-      tdot[k].s1 = 1.0;
-      tdot[k].out = dsc + k;
+      tres[k] = op_of(c);  // r = b - M^dag tmp, |r|^2
+      tres[k].d0 = r;
+      tres[k].d1 = (double *)b[c];
+      tres[k].d2 = (double *)K[c];
+      tres[k].d3 = tmp;
+      tres[k].s1 = 1.0;
+      tres[k].out = dsc + k;
     }
     const long nblk = th2d::bop_finish(tx);
-    for (auto *t : {&tm, &tmd, &tsub, &tdot}) th2d::bop_finish(*t);
+    for (auto *t : {&tm, &tres}) th2d::bop_finish(*t);
     std::vector<BOp> all(tx);
-    for (auto *t : {&tm, &tmd, &tsub, &tdot}) all.insert(all.end(), t->begin(), t->end());
+    for (auto *t : {&tm, &tres}) all.insert(all.end(), t->begin(), t->end());
     TH2D_TRY(th2d::bop_upload(ctx, ctx->btab[1], all));
     const BOp *dt = (const BOp *)ctx->btab[1].ptr;
     TH2D_TRY(th2d::bop_launch(ctx, mixed ? th2d::BOP_ADD_F32 : th2d::BOP_AXPY, dt, na, nblk));
     TH2D_TRY(th2d::bop_launch(ctx, th2d::BOP_DSLASH, dt + na, na, nblk));
-    TH2D_TRY(th2d::bop_launch(ctx, th2d::BOP_DSLASH, dt + 2 * na, na, nblk));
-    TH2D_TRY(th2d::bop_launch(ctx, th2d::BOP_SUB, dt + 3 * na, na, nblk));
-    TH2D_TRY(th2d::bop_launch(ctx, th2d::BOP_DOT, dt + 4 * na, na, nblk));
+    TH2D_TRY(th2d::bop_launch(ctx, th2d::BOP_RESID, dt + 2 * na, na, nblk));
     std::vector<double> rn(na);
     TH2D_TRY(th2d::fetch_doubles(ctx, rn.data(), dsc, na));
     for (int k = 0; k < na; k++) {

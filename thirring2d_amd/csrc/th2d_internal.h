@@ -394,6 +394,7 @@ enum {
   BOP_SCALE,       // d0 = s0 d1
   BOP_DOT,         // out = s1 * sum_sites Re(d0 . d1)  (two-pass, fixed order)
   BOP_COSGAP,      // out = s1 * sum_links (1 - cos d0)
+  BOP_RESID,       // d0 = d1 - M^dag d3 (K = d2); out = s1 * sum_sites |d0|^2
 };
 // Fills blk0 and returns the block count.
 long bop_finish(std::vector<BOp> &ops);
