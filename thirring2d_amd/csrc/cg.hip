@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -825,13 +826,16 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
   double gam_old = cont ? ch.gam0 * ch.cscale * ch.cscale : 1.0;
   double alpha_old = cont ? ch.alpha0 : 1.0;
   int slot = 0;
-  // iteration -1 (set-up): w_0 = A b
-  bool setup = true;
+  // iteration j; j = -1 is the set-up, w_0 = A b
   T ta = 0, tb = 0;
   V2 hr[NQ], hw[NQ], hs[NQ];  // halo values of the next iteration
-  for (int j = -1;; j++) {
+  // One iteration; false when the chain leaves.  The set-up is an instance
+  // of its own, so that in the loop every vector flows through one path (no
+  // register copies where a set-up path and an iteration path would join).
+  auto iteration = [&](auto setup_c, int j) __attribute__((always_inline)) -> bool {
+    constexpr bool setup = decltype(setup_c)::value;
     const int ns = setup ? 0 : slot ^ 1;  // parity written by this pass
-    if (!setup) {
+    if constexpr (!setup) {
       // 3a. s_j, r_{j+1} on the region and the halo; p_j, x_{j+1}.  Elements
       // outside the lattice part of the region (a region larger than what is
       // left of the lattice) first take the owners' values, in a pass of its
@@ -998,17 +1002,16 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
       }
     }
     slot = ns;
-    setup = false;
     // 1. every region of the chain has finished iteration j
     if (!M.sync(o_bar, (unsigned)(j + 2) * (unsigned)ntc, ctl.err, red + 36, ctl.zero, ctl.spin))
-      break;
+      return false;
     // 2. the chain's partials of iteration j+1, summed by every wave in the
     // same order (a butterfly leaves the total in every lane), so that all
     // waves of all regions of the chain hold bitwise the same gam, del; then
     // the halo layers of iteration j+1 (used from step 3a on: the sums only
     // wait for the partials)
     const int jn = j + 1;
-    const bool first_cont = jn == 0 && cont;
+    const bool first_cont = setup && cont;  // jn == 0
     double gam = 0.0, del = 0.0, rho = 0.0, sig = 0.0;
     for (int k = tid & 63; k < ntc * NW; k += 64) {
       gam += M.ldd(o_part[slot], 32u * k);
@@ -1040,7 +1043,7 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     else if (jn >= ch.max_iter) status = TH2D_ENOCONV;
     else if (!(del > 0.0) || !isfinite(gam)) status = TH2D_EBREAKDOWN;
     else {
-      const bool fresh = jn == 0 && !cont;
+      const bool fresh = setup && !cont;  // jn == 0
       beta = fresh ? 0.0 : gam / gam_old;
       // p.Ap from the CG relations, except after a residual replacement,
       // which breaks them: there p.Ap = r.w + 2 beta r.s + beta^2 p.s
@@ -1052,13 +1055,17 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     if (status != 1) {
       if (first && tid == 0)
         a.res[tl.chain] = CgResult{jn, status, gam, 1, 0, gam_old, alpha_old};
-      break;
+      return false;
     }
     gam_old = gam;
     alpha_old = alpha;
     ta = (T)alpha;
     tb = (T)beta;
-  }
+    return true;
+  };
+  if (iteration(std::true_type{}, -1))
+    for (int j = 0; iteration(std::false_type{}, j); j++) {
+    }
   // x of the owned sites, and p, s for a reliable update
   V2 *xo = a.x + ch.voff;
 #pragma unroll
