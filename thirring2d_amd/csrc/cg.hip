@@ -464,6 +464,7 @@ using RShapeH = Shape<32, 64, 512>;
 static_assert(RShape::TT == RES_T && RShape::TX == RES_T && RShapeH::TX == RES_T,
               "regions of the publication layout");
 constexpr unsigned RES_SPIN_LIMIT = 1u << 20;            // polls of one wait (about 1 s)
+constexpr int RES_PART_LANE = 4;  // partials a lane reads: a chain has at most 256 (host check)
 constexpr unsigned kNoLoad = 0xFFFFFFFFu;                 // halo element no result needs
 
 struct ResCtl {
@@ -680,6 +681,39 @@ __device__ __forceinline__ double wave_total(double v)
   return __longlong_as_double(((long long)hi << 32) | lo);
 }
 
+// The same for a float, each step one add with a DPP operand (the result
+// is the float total in a fixed order, widened).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_f(float v)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+__device__ __forceinline__ double wave_total_f(float v)
+{
+  v += dpp_f<0xB1, 0xf>(v);
+  v += dpp_f<0x4E, 0xf>(v);
+  v += dpp_f<0x141, 0xf>(v);
+  v += dpp_f<0x140, 0xf>(v);
+  v += dpp_f<0x142, 0xa>(v);
+  v += dpp_f<0x143, 0xc>(v);
+  return (double)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// a / b from the hardware reciprocal refined by two Newton steps and a
+// corrected quotient (a few fused multiply-adds instead of the scaled
+// division sequence).  Every region computes it bitwise alike; b = 0 gives
+// NaN, which the breakdown checks catch as they catch an infinite quotient.
+__device__ __forceinline__ double qdiv(double a, double b)
+{
+  double r = __builtin_amdgcn_rcp(b);
+  double e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-b, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  const double q = a * r;
+  return __builtin_fma(r, __builtin_fma(-b, q, a), q);
+}
+
 // XCD of the calling workgroup (hardware register XCC_ID, bits 3:0).
 __device__ __forceinline__ int xcc_id()
 {
@@ -791,16 +825,22 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
       s[q] = cscale(*at((const V2 *)ch.cs, oi), cf);
     }
   }
-  unsigned ri[NQ];  // publication offsets of the halo elements (kNoLoad: none)
+  // A buffer load at an offset at or past the end of the block (nol) is out
+  // of the descriptor's range and returns 0 without a memory access: the
+  // loads below are unconditional, so that the wait for the first of them
+  // does not also wait for the later ones.
+  const unsigned nol = ctl.blk_bytes;
+  unsigned ri[NQ];  // publication offsets of the halo elements (nol: none)
   int re[NQ];       // their places in R2
 #pragma unroll
   for (int h = 0; h < NQ; h++) {
-    ri[h] = kNoLoad;
+    ri[h] = nol;
     re[h] = 0;
     if (tid + h * NT < NR) {
       int et, ex;
       res_ring_pos<S>(tid + h * NT, et, ex);
-      ri[h] = src_at(et - 2, ex - 2);
+      const unsigned o = src_at(et - 2, ex - 2);
+      ri[h] = o == kNoLoad ? nol : o;
       re[h] = et * W2 + ex;
     }
   }
@@ -985,11 +1025,11 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     // reduction (LDS round trip and barrier) is needed here
     const int npart = setup && cont ? 4 : 2;
     {
-      const double g = wave_total((double)gacc32), d = wave_total((double)dacc32);
+      const double g = wave_total_f(gacc32), d = wave_total_f(dacc32);
       double rs = 0.0, ps = 0.0;
       if (npart == 4) {
-        rs = wave_total((double)racc32);
-        ps = wave_total((double)sacc32);
+        rs = wave_total_f(racc32);
+        ps = wave_total_f(sacc32);
       }
       if ((tid & 63) == 0) {
         const unsigned mp = my_part + 32u * (unsigned)(tid >> 6);
@@ -1012,23 +1052,36 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     // wait for the partials)
     const int jn = j + 1;
     const bool first_cont = setup && cont;  // jn == 0
-    double gam = 0.0, del = 0.0, rho = 0.0, sig = 0.0;
-    for (int k = tid & 63; k < ntc * NW; k += 64) {
-      gam += M.ldd(o_part[slot], 32u * k);
-      del += M.ldd(o_part[slot], 32u * k + 8);
+    // Every load is issued before the first wait: the partials (at most
+    // RES_PART_LANE per lane), then the halo values, which wait until 3a.
+    const int npt = ntc * NW;
+    double pg[RES_PART_LANE], pd[RES_PART_LANE], pr[RES_PART_LANE], ps[RES_PART_LANE];
+#pragma unroll
+    for (int u = 0; u < RES_PART_LANE; u++) {
+      const int k = (tid & 63) + 64 * u;
+      const unsigned po = k < npt ? 32u * (unsigned)k : nol;  // nol: 0
+      pg[u] = M.ldd(o_part[slot], po);
+      pd[u] = M.ldd(o_part[slot], po + 8);
+      pr[u] = ps[u] = 0.0;
       if (first_cont) {
-        rho += M.ldd(o_part[slot], 32u * k + 16);
-        sig += M.ldd(o_part[slot], 32u * k + 24);
+        pr[u] = M.ldd(o_part[slot], po + 16);
+        ps[u] = M.ldd(o_part[slot], po + 24);
       }
     }
 #pragma unroll
-    for (int h = 0; h < NQ; h++) {
-      if (tid + h * NT >= NR) continue;
-      hr[h] = hw[h] = hs[h] = zero;
-      if (ri[h] != kNoLoad) {
-        hr[h] = M.ld2(o_r[slot], ri[h]);
-        hw[h] = M.ld2(o_w[slot], ri[h]);
-        hs[h] = M.ld2(o_s[slot], ri[h]);
+    for (int h = 0; h < NQ; h++) {  // nol: 0
+      hr[h] = M.ld2(o_r[slot], ri[h]);
+      hw[h] = M.ld2(o_w[slot], ri[h]);
+      hs[h] = M.ld2(o_s[slot], ri[h]);
+    }
+    double gam = 0.0, del = 0.0, rho = 0.0, sig = 0.0;  // a lane's partials in order
+#pragma unroll
+    for (int u = 0; u < RES_PART_LANE; u++) {
+      gam += pg[u];
+      del += pd[u];
+      if (first_cont) {
+        rho += pr[u];
+        sig += ps[u];
       }
     }
     gam = wave_total(gam);
@@ -1044,12 +1097,12 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     else if (!(del > 0.0) || !isfinite(gam)) status = TH2D_EBREAKDOWN;
     else {
       const bool fresh = setup && !cont;  // jn == 0
-      beta = fresh ? 0.0 : gam / gam_old;
+      beta = fresh ? 0.0 : qdiv(gam, gam_old);
       // p.Ap from the CG relations, except after a residual replacement,
       // which breaks them: there p.Ap = r.w + 2 beta r.s + beta^2 p.s
-      alpha = fresh ? gam / del
-                    : gam / (first_cont ? del + 2.0 * beta * rho + beta * beta * sig
-                                        : del - beta * gam / alpha_old);
+      alpha = fresh ? qdiv(gam, del)
+                    : qdiv(gam, first_cont ? del + 2.0 * beta * rho + beta * beta * sig
+                                           : del - qdiv(beta * gam, alpha_old));
       if (!(alpha > 0.0) || !isfinite(alpha)) status = TH2D_EBREAKDOWN;
     }
     if (status != 1) {
@@ -2049,7 +2102,9 @@ int resident_solve_s(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   const int cap = res_capacity(ctx);
   if (nreg < 1 || cap < kNumXcd) return TH2D_ELAUNCH;
   for (int c = 0; c < nch; c++)
-    if (!res_eligible(G[c], S::TT)) return TH2D_ELAUNCH;
+    if (!res_eligible(G[c], S::TT) ||
+        res_chain_regions(G[c], S::TT) * (S::NT / 64) > 64 * RES_PART_LANE)
+      return TH2D_ELAUNCH;
   std::vector<int> wg_tile;
   std::vector<ResGroup> groups;
   if (!place_regions(nch, G, cap / kNumXcd, S::TT, wg_tile, groups)) return TH2D_ELAUNCH;
