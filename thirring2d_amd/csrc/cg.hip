@@ -721,8 +721,12 @@ __device__ __forceinline__ int xcc_id()
 }
 
 // MU0: every chain of the batch has mu = 0 (the e^{+-mu} factors are 1);
-// S: the region shape (RShape or RShapeH)
-template <typename T, bool MU0, typename S>
+// S: the region shape (RShape or RShapeH); CLIP: some region of the batch
+// is larger than what is left of its lattice (a lattice of one region
+// smaller than the shape).  Without clipped regions every element is owned
+// and no load of another region's values precedes the halo's: the waits of
+// 3a are then for the halo loads alone.
+template <typename T, bool MU0, typename S, bool CLIP>
 __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
 {
   using V2 = c2<T>;
@@ -806,16 +810,25 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     return (unsigned)eo_index(G, wrapn(tl.t0 + lt, G.nt), wrapn(tl.x0 + lx, G.nx)) *
            (unsigned)sizeof(V2);
   };
-  unsigned inl = 0, pub = 0;
+  // A buffer access at an offset at or past the end of the block (nol) is
+  // out of the descriptor's range: a load returns 0 and a store is dropped,
+  // without a memory access.  The publication stores and the halo and
+  // partial loads are unconditional (nol where an element has none), so that
+  // a wait for a load counts exactly the accesses issued after it and does
+  // not wait for later stores.
+  const unsigned nol = ctl.blk_bytes;
+  unsigned inl = 0;
+  unsigned pso[NO];  // publication offset of owned element q (nol: none)
   V2 r[NO], w[NO], s[NO], p[NO], x[NO];
   const V2 zero = cmk<V2>((T)0, (T)0);
 #pragma unroll
   for (int q = 0; q < NO; q++) {
     const int lt = wrow + q, lx = lane;
     const unsigned oi = oi_of(q);
+    pso[q] = nol;
     if (lt < tte && lx < txe) {
       inl |= 1u << q;
-      if (lt < 2 || lt >= tte - 2 || lx < 2 || lx >= txe - 2) pub |= 1u << q;
+      if (lt < 2 || lt >= tte - 2 || lx < 2 || lx >= txe - 2) pso[q] = pub_at(lt, lx);
     }
     r[q] = *at(b, oi);
     w[q] = s[q] = p[q] = x[q] = zero;
@@ -825,11 +838,6 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
       s[q] = cscale(*at((const V2 *)ch.cs, oi), cf);
     }
   }
-  // A buffer load at an offset at or past the end of the block (nol) is out
-  // of the descriptor's range and returns 0 without a memory access: the
-  // loads below are unconditional, so that the wait for the first of them
-  // does not also wait for the later ones.
-  const unsigned nol = ctl.blk_bytes;
   unsigned ri[NQ];  // publication offsets of the halo elements (nol: none)
   int re[NQ];       // their places in R2
 #pragma unroll
@@ -885,7 +893,7 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
       // before the halo ring: it does not wait for the halo loads.
 #pragma unroll
       for (int q = 0; q < NO; q++) {
-        if ((inl >> q) & 1u) continue;
+        if (!CLIP || ((inl >> q) & 1u)) continue;
         const int lt = wrow + q, lx = lane;
         const unsigned so = src_at(lt, lx);
         r[q] = w[q] = s[q] = zero;
@@ -898,7 +906,7 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
 #pragma unroll
       for (int q = 0; q < NO; q++) {
         const int lt = wrow + q, lx = lane;
-        const bool own = (inl >> q) & 1u;
+        const bool own = !CLIP || ((inl >> q) & 1u);
         const V2 sn = cadd(w[q], cscale(s[q], tb));
         const V2 rn = csub(r[q], cscale(sn, ta));
         R2[(lt + 2) * W2 + (lx + 2)] = rn;
@@ -908,11 +916,8 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
         }
         r[q] = rn;
         s[q] = sn;
-        if ((pub >> q) & 1u) {
-          const unsigned po = pub_at(lt, lx);
-          M.st2(o_r[ns], po, rn);
-          M.st2(o_s[ns], po, sn);
-        }
+        M.st2(o_r[ns], pso[q], rn);  // nol: dropped
+        M.st2(o_s[ns], pso[q], sn);
       }
 #pragma unroll
       for (int h = 0; h < NQ; h++) {
@@ -922,13 +927,10 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
       }
     } else {
 #pragma unroll
-      for (int q = 0; q < NO; q++)
-        if ((pub >> q) & 1u) {
-          const int lt = wrow + q, lx = lane;
-          const unsigned po = pub_at(lt, lx);
-          M.st2(o_r[0], po, r[q]);
-          M.st2(o_s[0], po, s[q]);
-        }
+      for (int q = 0; q < NO; q++) {
+        M.st2(o_r[0], pso[q], r[q]);  // nol: dropped
+        M.st2(o_s[0], pso[q], s[q]);
+      }
     }
     __syncthreads();
     // 3b. y = M r_{j+1} on the 1-halo (LDS only).  A wave takes the NO
@@ -996,7 +998,6 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
       V2 ytm = Y1[c1 - W1], y0 = Y1[c1], ktd = LKt[c2i - W2];
 #pragma unroll
       for (int q = 0; q < NO; q++, c1 += W1, c2i += W2) {
-        const int lt = wrow + q, lx = lane;
         const V2 kt = LKt[c2i], kx = LKx[c2i], kxd = LKx[c2i - 1];
         const V2 ytp = Y1[c1 + W1], yxp = Y1[c1 + 1], yxm = Y1[c1 - 1];
         __builtin_amdgcn_sched_barrier(0);  // one LDS latency per element (as in 3b)
@@ -1009,9 +1010,9 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
         ytm = y0;
         y0 = ytp;
         ktd = kt;
-        if (!((inl >> q) & 1u)) continue;
+        if (CLIP && !((inl >> q) & 1u)) continue;
         w[q] = v;
-        if ((pub >> q) & 1u) M.st2(o_w[ns], pub_at(lt, lx), v);
+        M.st2(o_w[ns], pso[q], v);  // nol: dropped
         gacc32 += r[q].x * r[q].x + r[q].y * r[q].y;
         dacc32 += r[q].x * v.x + r[q].y * v.y;
         if (setup && cont) {
@@ -1123,7 +1124,7 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
   V2 *xo = a.x + ch.voff;
 #pragma unroll
   for (int q = 0; q < NO; q++)
-    if ((inl >> q) & 1u) {
+    if (!CLIP || ((inl >> q) & 1u)) {
       const unsigned oi = oi_of(q);
       *at(xo, oi) = x[q];
       if (ch.cp) {
@@ -2013,9 +2014,10 @@ int res_capacity(th2d_ctx *ctx)
   if (ctx->res_capacity < 0) {
     int occ = 0, occh = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &occ, (const void *)k_cg_resident<float, true, RShape>, RShape::NT, 0) != hipSuccess ||
+            &occ, (const void *)k_cg_resident<float, true, RShape, true>, RShape::NT, 0) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &occh, (const void *)k_cg_resident<float, true, RShapeH>, RShapeH::NT, 0) != hipSuccess)
+            &occh, (const void *)k_cg_resident<float, true, RShapeH, true>, RShapeH::NT, 0) !=
+            hipSuccess)
       occ = occh = 0;
     ctx->res_capacity = std::min(occ, occh) * ctx->num_cu;
   }
@@ -2153,15 +2155,20 @@ int resident_solve_s(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   hipLaunchKernelGGL(k_zero_u32, dim3(std::min<int>(64, (int)(nzero + 255) / 256)), dim3(256), 0,
                      ctx->stream, ctl.bar, (int)nzero);
   TH2D_HIP(hipGetLastError());
-  bool mu0 = true;
-  for (int c = 0; c < nch; c++) mu0 &= G[c].ep == 1.0 && G[c].em == 1.0;
+  bool mu0 = true, clip = false;
+  for (int c = 0; c < nch; c++) {
+    mu0 &= G[c].ep == 1.0 && G[c].em == 1.0;
+    clip |= G[c].nt % S::TT != 0 || G[c].nx % S::TX != 0;
+  }
+  const void *kern = mu0 ? (clip ? (const void *)k_cg_resident<float, true, S, true>
+                                 : (const void *)k_cg_resident<float, true, S, false>)
+                         : (clip ? (const void *)k_cg_resident<float, false, S, true>
+                                 : (const void *)k_cg_resident<float, false, S, false>);
   const int *tab0 = ctl.wg_tile;
   for (const ResGroup &g : groups) {  // one launch per group (each chain in one group)
     ctl.wg_tile = tab0 + g.tab;
     void *args[] = {(void *)&a, (void *)&ctl};
-    TH2D_HIP(hipLaunchCooperativeKernel(
-        mu0 ? k_cg_resident<float, true, S> : k_cg_resident<float, false, S>, dim3(g.grid),
-        dim3(S::NT), args, 0, ctx->stream));
+    TH2D_HIP(hipLaunchCooperativeKernel(kern, dim3(g.grid), dim3(S::NT), args, 0, ctx->stream));
   }
   // results and the two flags into one pinned block, then one wait
   const size_t rb = sizeof(CgResult) * nch;
