@@ -464,7 +464,7 @@ using RShapeH = Shape<32, 64, 512>;
 static_assert(RShape::TT == RES_T && RShape::TX == RES_T && RShapeH::TX == RES_T,
               "regions of the publication layout");
 constexpr unsigned RES_SPIN_LIMIT = 1u << 20;            // polls of one wait (about 1 s)
-constexpr int RES_PART_LANE = 4;  // partials a lane reads: a chain has at most 256 (host check)
+constexpr int RES_PART_LANE = 4;  // partial slots a lane reads: at most 256 a chain (host check)
 constexpr unsigned kNoLoad = 0xFFFFFFFFu;                 // halo element no result needs
 
 struct ResCtl {
@@ -475,8 +475,8 @@ struct ResCtl {
   int *misplaced;    // set when a chain's regions do not share an XCD
   int *xcc;          // [ntiles] XCD of the workgroup of each region
   double *part;      // [2][nparts][waves][4] partials; a chain's start on a 2-KB boundary
-  void *pub;         // [2][3][ntiles][RES_PUB] published boundary layers of r, s, w
-                     // per parity (res_layout.h)
+  void *pub;         // [2][2][ntiles][RES_PUB] published boundary layers per parity
+                     // (res_layout.h): (r, s) pairs, then w (16 B per slot each)
   int nparts;        // partial slots per parity
   unsigned blk_bytes;  // bytes of the solver block from CgBatch::x (buffer range)
   unsigned zero;     // 0; an addend the compiler cannot see, so that a poll
@@ -530,6 +530,9 @@ __device__ __forceinline__ void l1_invalidate()
 // compiler turns an atomic add of a known 0 into such a load), then an L1
 // invalidation before the reads (loads with cache policy sc0).
 typedef unsigned b64v __attribute__((ext_vector_type(2)));
+typedef unsigned b128v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
 struct ResMem {
   char *base;
   __amdgpu_buffer_rsrc_t rs;
@@ -541,6 +544,23 @@ struct ResMem {
   __device__ __forceinline__ void st2(unsigned soff, unsigned voff, float2 v) const
   {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(b64v, v), rs, voff, soff, 0);
+  }
+  // 16 bytes: two FP32 complex values, or two doubles
+  __device__ __forceinline__ f4v ld4(unsigned soff, unsigned voff) const
+  {
+    return __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 1));
+  }
+  __device__ __forceinline__ d2v ldd2(unsigned soff, unsigned voff) const
+  {
+    return __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 1));
+  }
+  __device__ __forceinline__ void st4(unsigned soff, unsigned voff, f4v v) const
+  {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(b128v, v), rs, voff, soff, 0);
+  }
+  __device__ __forceinline__ void std2(unsigned soff, unsigned voff, d2v v) const
+  {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(b128v, v), rs, voff, soff, 0);
   }
   __device__ __forceinline__ double ldd(unsigned soff, unsigned voff) const
   {
@@ -763,16 +783,16 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
   M.base = (char *)a.x;
   M.rs = __builtin_amdgcn_make_buffer_rsrc(a.x, 0, (int)ctl.blk_bytes, 0x00020000);
   auto boff = [&](const void *p) { return (unsigned)((const char *)p - M.base); };
-  // publication planes of r, s, w per parity, and this region's block
-  constexpr unsigned PB = RES_PUB * sizeof(V2);
+  // publication planes per parity, (r, s) pairs and w, 16 B per slot (one
+  // load or store for r and s), and this region's block
+  constexpr unsigned PS = 16, PB = RES_PUB * PS;
   const unsigned plane = (unsigned)a.ntiles * PB, o_pub = boff(ctl.pub);
-  const unsigned o_r[2] = {o_pub, o_pub + 3 * plane};
-  const unsigned o_s[2] = {o_pub + plane, o_pub + 4 * plane};
-  const unsigned o_w[2] = {o_pub + 2 * plane, o_pub + 5 * plane};
+  const unsigned o_rs[2] = {o_pub, o_pub + 2 * plane};
+  const unsigned o_w[2] = {o_pub + plane, o_pub + 3 * plane};
   const unsigned my_blk = (unsigned)tau * PB;
   // publication offset of owned element (lt, lx) within 2 sites of the edge
   auto pub_at = [&](int lt, int lx) {
-    return my_blk + (unsigned)res_pub_slot(lt, lx, tte, txe) * (unsigned)sizeof(V2);
+    return my_blk + (unsigned)res_pub_slot(lt, lx, tte, txe) * PS;
   };
   // publication offset of the site at region coordinates (lt, lx) outside
   // the owned block, or kNoLoad when no owned result depends on it
@@ -782,7 +802,7 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
       int k;
       const int sl =
           res_site_slot(wrapn(tl.t0 + lt, G.nt), wrapn(tl.x0 + lx, G.nx), G.nt, G.nx, k, TT);
-      if (sl >= 0) off = (unsigned)(ch.tbeg + k) * PB + (unsigned)sl * (unsigned)sizeof(V2);
+      if (sl >= 0) off = (unsigned)(ch.tbeg + k) * PB + (unsigned)sl * PS;
     }
     return off;
   };
@@ -898,9 +918,10 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
         const unsigned so = src_at(lt, lx);
         r[q] = w[q] = s[q] = zero;
         if (so != kNoLoad) {
-          r[q] = M.ld2(o_r[slot], so);
+          const f4v v = M.ld4(o_rs[slot], so);
+          r[q] = cmk<V2>(v.x, v.y);
+          s[q] = cmk<V2>(v.z, v.w);
           w[q] = M.ld2(o_w[slot], so);
-          s[q] = M.ld2(o_s[slot], so);
         }
       }
 #pragma unroll
@@ -916,8 +937,7 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
         }
         r[q] = rn;
         s[q] = sn;
-        M.st2(o_r[ns], pso[q], rn);  // nol: dropped
-        M.st2(o_s[ns], pso[q], sn);
+        M.st4(o_rs[ns], pso[q], f4v{rn.x, rn.y, sn.x, sn.y});  // nol: dropped
       }
 #pragma unroll
       for (int h = 0; h < NQ; h++) {
@@ -928,8 +948,7 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     } else {
 #pragma unroll
       for (int q = 0; q < NO; q++) {
-        M.st2(o_r[0], pso[q], r[q]);  // nol: dropped
-        M.st2(o_s[0], pso[q], s[q]);
+        M.st4(o_rs[0], pso[q], f4v{r[q].x, r[q].y, s[q].x, s[q].y});  // nol: dropped
       }
     }
     __syncthreads();
@@ -1034,12 +1053,8 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
       }
       if ((tid & 63) == 0) {
         const unsigned mp = my_part + 32u * (unsigned)(tid >> 6);
-        M.std_(o_part[ns], mp, g);
-        M.std_(o_part[ns], mp + 8, d);
-        if (npart == 4) {
-          M.std_(o_part[ns], mp + 16, rs);
-          M.std_(o_part[ns], mp + 24, ps);
-        }
+        M.std2(o_part[ns], mp, d2v{g, d});
+        if (npart == 4) M.std2(o_part[ns], mp + 16, d2v{rs, ps});
       }
     }
     slot = ns;
@@ -1053,36 +1068,40 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     // wait for the partials)
     const int jn = j + 1;
     const bool first_cont = setup && cont;  // jn == 0
-    // Every load is issued before the first wait: the partials (at most
-    // RES_PART_LANE per lane), then the halo values, which wait until 3a.
+    // Every load is issued before the first wait: the partials (gam, del of
+    // RES_PART_LANE slots per lane, 0 past the chain's), the halo values of
+    // the first ring element, then in a wave-uniform branch those of the
+    // other ring elements.  A wait for a load then never waits for the halo
+    // loads issued after it.
     const int npt = ntc * NW;
-    double pg[RES_PART_LANE], pd[RES_PART_LANE], pr[RES_PART_LANE], ps[RES_PART_LANE];
+    const int wave0 = __builtin_amdgcn_readfirstlane(tid & ~63);
+    d2v pgd[RES_PART_LANE], prs[RES_PART_LANE];
 #pragma unroll
     for (int u = 0; u < RES_PART_LANE; u++) {
+      prs[u] = d2v{0.0, 0.0};
       const int k = (tid & 63) + 64 * u;
       const unsigned po = k < npt ? 32u * (unsigned)k : nol;  // nol: 0
-      pg[u] = M.ldd(o_part[slot], po);
-      pd[u] = M.ldd(o_part[slot], po + 8);
-      pr[u] = ps[u] = 0.0;
-      if (first_cont) {
-        pr[u] = M.ldd(o_part[slot], po + 16);
-        ps[u] = M.ldd(o_part[slot], po + 24);
-      }
+      pgd[u] = M.ldd2(o_part[slot], po);
+      if (first_cont) prs[u] = M.ldd2(o_part[slot], po + 16);
     }
 #pragma unroll
-    for (int h = 0; h < NQ; h++) {  // nol: 0
-      hr[h] = M.ld2(o_r[slot], ri[h]);
-      hw[h] = M.ld2(o_w[slot], ri[h]);
-      hs[h] = M.ld2(o_s[slot], ri[h]);
+    for (int h = 0; h < NQ; h++) {
+      hr[h] = hw[h] = hs[h] = zero;
+      if (h == 0 || wave0 + h * NT < NR) {  // (uniform)
+        const f4v v = M.ld4(o_rs[slot], ri[h]);  // nol: 0
+        hr[h] = cmk<V2>(v.x, v.y);
+        hs[h] = cmk<V2>(v.z, v.w);
+        hw[h] = M.ld2(o_w[slot], ri[h]);
+      }
     }
     double gam = 0.0, del = 0.0, rho = 0.0, sig = 0.0;  // a lane's partials in order
 #pragma unroll
     for (int u = 0; u < RES_PART_LANE; u++) {
-      gam += pg[u];
-      del += pd[u];
+      gam += pgd[u].x;
+      del += pgd[u].y;
       if (first_cont) {
-        rho += pr[u];
-        sig += ps[u];
+        rho += prs[u].x;
+        sig += prs[u].y;
       }
     }
     gam = wave_total(gam);
@@ -2126,7 +2145,7 @@ int resident_solve_s(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   const size_t ntab = (wg_tile.size() + 31) & ~(size_t)31;
   const size_t nzero = 64 * (size_t)nch + 32;  // counters and error flags, zeroed
   const size_t nxcc = ((size_t)nreg + 31) & ~(size_t)31;
-  const size_t pub_bytes = 6 * (size_t)nreg * RES_PUB * sizeof(c2<float>);  // 2 x (r, s, w)
+  const size_t pub_bytes = 4 * (size_t)nreg * RES_PUB * 16;  // 2 x ((r, s), w), 16 B a slot
   constexpr size_t NW = S::NT / 64;  // partial slots of a region: one per wave
   const size_t extra_bytes = 4 * (ntab + nzero + nxcc) + 64 * NW * (size_t)nparts + pub_bytes;
   CgBatch<float> a;
