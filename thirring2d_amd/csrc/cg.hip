@@ -960,15 +960,25 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     // registers.  Then the frame's first and last rows and columns, at most
     // one element per thread.
     {
-      int c = (wrow + 2) * W2 + (lane + 2);  // R2 index of 1-halo (wrow + 1, lane + 1)
+      const int c = (wrow + 2) * W2 + (lane + 2);  // R2 index of 1-halo (wrow + 1, lane + 1)
       V2 rtm = R2[c - W2], r0 = R2[c], ktd = LKt[c - W2];
-      V2 *yo = Y1 + (wrow + 1) * W1 + (lane + 1);
-#pragma unroll 1
-      for (int q = 0; q < NO; q++, c += W2, yo += W1) {
-        const V2 kt = LKt[c], kx = LKx[c], kxd = LKx[c - 1];
-        const V2 rtp = R2[c + W2], rxp = R2[c + 1], rxm = R2[c - 1];
-        // all loads in flight before the arithmetic: one LDS latency per
-        // element (two waves per SIMD hide little)
+      V2 *const yo = Y1 + (wrow + 1) * W1 + (lane + 1);
+      // unrolled, one element ahead: the loads of element q + 1 are issued
+      // before the arithmetic of element q, which waits for its own loads only
+      V2 kt = LKt[c], kx = LKx[c], kxd = LKx[c - 1];
+      V2 rtp = R2[c + W2], rxp = R2[c + 1], rxm = R2[c - 1];
+#pragma unroll
+      for (int q = 0; q < NO; q++) {
+        const int cn = c + (q + 1) * W2;
+        V2 nkt, nkx, nkxd, nrtp, nrxp, nrxm;
+        if (q + 1 < NO) {
+          nkt = LKt[cn];
+          nkx = LKx[cn];
+          nkxd = LKx[cn - 1];
+          nrtp = R2[cn + W2];
+          nrxp = R2[cn + 1];
+          nrxm = R2[cn - 1];
+        }
         __builtin_amdgcn_sched_barrier(0);
         // y = m r + e^mu K_t r(t+1) - e^-mu K_t(t-1)* r(t-1) + K_x r(x+1) - K_x(x-1)* r(x-1)
         V2 y = cscale(r0, m);
@@ -976,10 +986,19 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
         cjfma(y, ktd, rtm, -emm);
         cfma(y, kx, rxp, 1.0f);
         cjfma(y, kxd, rxm, -1.0f);
-        *yo = y;
+        yo[q * W1] = y;
         rtm = r0;
         r0 = rtp;
         ktd = kt;
+        if (q + 1 < NO) {
+          kt = nkt;
+          kx = nkx;
+          kxd = nkxd;
+          rtp = nrtp;
+          rxp = nrxp;
+          rxm = nrxm;
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
       for (int e = tid; e < 2 * W1 + 2 * TT; e += NT) {
         int ft, fx;
