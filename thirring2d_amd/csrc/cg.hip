@@ -897,6 +897,7 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
   // iteration j; j = -1 is the set-up, w_0 = A b
   T ta = 0, tb = 0;
   V2 hr[NQ], hw[NQ], hs[NQ];  // halo values of the next iteration
+  V2 yv[NO];                  // y = M r on the wave's rows (3b, used in 3c)
   // One iteration; false when the chain leaves.  The set-up is an instance
   // of its own, so that in the loop every vector flows through one path (no
   // register copies where a set-up path and an iteration path would join).
@@ -952,34 +953,35 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
       }
     }
     __syncthreads();
-    // 3b. y = M r_{j+1} on the 1-halo (LDS only).  A wave takes the NO
-    // rows wrow + 1 .. wrow + NO of the 1-halo frame at column lane + 1;
-    // going down the rows, r(t-1), r(t) and the link K_t(t-1) come from the
-    // previous row's registers (6 LDS loads per element instead of 9).  The
-    // loop is not unrolled, which would hoist every row's loads into
-    // registers.  Then the frame's first and last rows and columns, at most
-    // one element per thread.
+    // 3b. y = M r_{j+1} on the 1-halo.  A wave takes the NO rows wrow + 1 ..
+    // wrow + NO of the 1-halo frame at column lane + 1, its own sites: r(t),
+    // r(t +- 1) come from r[] and K_t(t-1) from the previous row, so an
+    // element loads its two x neighbours and three links from LDS (the
+    // rows above and below the wave's once).  Then the frame's first and
+    // last rows and columns, at most one element per thread.
     {
       const int c = (wrow + 2) * W2 + (lane + 2);  // R2 index of 1-halo (wrow + 1, lane + 1)
-      V2 rtm = R2[c - W2], r0 = R2[c], ktd = LKt[c - W2];
+      // r on the wave's rows is r[] (3a wrote the same values to R2); the
+      // rows above and below come from LDS
+      const V2 rtop = R2[c - W2], rbot = R2[c + NO * W2];
+      V2 ktd = LKt[c - W2];
       V2 *const yo = Y1 + (wrow + 1) * W1 + (lane + 1);
       // unrolled, one element ahead: the loads of element q + 1 are issued
       // before the arithmetic of element q, which waits for its own loads only
-      V2 kt = LKt[c], kx = LKx[c], kxd = LKx[c - 1];
-      V2 rtp = R2[c + W2], rxp = R2[c + 1], rxm = R2[c - 1];
+      V2 kt = LKt[c], kx = LKx[c], kxd = LKx[c - 1], rxp = R2[c + 1], rxm = R2[c - 1];
 #pragma unroll
       for (int q = 0; q < NO; q++) {
         const int cn = c + (q + 1) * W2;
-        V2 nkt, nkx, nkxd, nrtp, nrxp, nrxm;
+        V2 nkt, nkx, nkxd, nrxp, nrxm;
         if (q + 1 < NO) {
           nkt = LKt[cn];
           nkx = LKx[cn];
           nkxd = LKx[cn - 1];
-          nrtp = R2[cn + W2];
           nrxp = R2[cn + 1];
           nrxm = R2[cn - 1];
         }
         __builtin_amdgcn_sched_barrier(0);
+        const V2 r0 = r[q], rtp = q + 1 < NO ? r[q + 1] : rbot, rtm = q > 0 ? r[q - 1] : rtop;
         // y = m r + e^mu K_t r(t+1) - e^-mu K_t(t-1)* r(t-1) + K_x r(x+1) - K_x(x-1)* r(x-1)
         V2 y = cscale(r0, m);
         cfma(y, kt, rtp, epm);
@@ -987,14 +989,12 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
         cfma(y, kx, rxp, 1.0f);
         cjfma(y, kxd, rxm, -1.0f);
         yo[q * W1] = y;
-        rtm = r0;
-        r0 = rtp;
+        yv[q] = y;
         ktd = kt;
         if (q + 1 < NO) {
           kt = nkt;
           kx = nkx;
           kxd = nkxd;
-          rtp = nrtp;
           rxp = nrxp;
           rxm = nrxm;
         }
@@ -1025,28 +1025,30 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     }
     __syncthreads();
     // 3c. w_{j+1} = M^dag y on the region; partials of iteration j+1 (a
-    // thread's sites in FP32, then FP64).  Going down the wave's rows, y(t-1),
-    // y(t) and K_t(t-1) come from the previous row's registers.  Elements
+    // thread's sites in FP32, then FP64).  y(t), y(t +- 1) come from yv[]
+    // (the rows above and below the wave's from LDS), K_t(t-1) from the
+    // previous row.  Elements
     // outside the lattice (a region larger than the lattice) are computed
     // too, to keep that chain, and their results dropped.
     T gacc32 = 0, dacc32 = 0, racc32 = 0, sacc32 = 0;
     {
       int c1 = (wrow + 1) * W1 + (lane + 1);   // Y1 index of (wrow, lane)
       int c2i = (wrow + 2) * W2 + (lane + 2);  // R2 / link index of (wrow, lane)
-      V2 ytm = Y1[c1 - W1], y0 = Y1[c1], ktd = LKt[c2i - W2];
+      // y on the wave's rows is yv[] (3b); the rows above and below from LDS
+      const V2 ytop = Y1[c1 - W1], ybot = Y1[c1 + NO * W1];
+      V2 ktd = LKt[c2i - W2];
 #pragma unroll
       for (int q = 0; q < NO; q++, c1 += W1, c2i += W2) {
         const V2 kt = LKt[c2i], kx = LKx[c2i], kxd = LKx[c2i - 1];
-        const V2 ytp = Y1[c1 + W1], yxp = Y1[c1 + 1], yxm = Y1[c1 - 1];
+        const V2 yxp = Y1[c1 + 1], yxm = Y1[c1 - 1];
         __builtin_amdgcn_sched_barrier(0);  // one LDS latency per element (as in 3b)
+        const V2 y0 = yv[q], ytp = q + 1 < NO ? yv[q + 1] : ybot, ytm = q > 0 ? yv[q - 1] : ytop;
         // w = m y - e^-mu K_t y(t+1) + e^mu K_t(t-1)* y(t-1) - K_x y(x+1) + K_x(x-1)* y(x-1)
         V2 v = cscale(y0, m);
         cfma(v, kt, ytp, -emm);
         cjfma(v, ktd, ytm, epm);
         cfma(v, kx, yxp, -1.0f);
         cjfma(v, kxd, yxm, 1.0f);
-        ytm = y0;
-        y0 = ytp;
         ktd = kt;
         if (CLIP && !((inl >> q) & 1u)) continue;
         w[q] = v;
