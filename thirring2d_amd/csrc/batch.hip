@@ -125,9 +125,7 @@ int bop_upload(th2d_ctx *c, th2d_buf &dev, const std::vector<BOp> &ops)
 {
   const size_t bytes = sizeof(BOp) * ops.size();
   TH2D_TRY(th2d_buf_reserve(dev, bytes));
-  // pageable source: the copy has staged the data when the call returns
-  TH2D_HIP(hipMemcpyAsync(dev.ptr, ops.data(), bytes, hipMemcpyHostToDevice, c->stream));
-  return TH2D_OK;
+  return stage_h2d(c, dev.ptr, ops.data(), bytes);
 }
 
 int bop_launch(th2d_ctx *c, int op, const BOp *dev, int nch, long nblk)

@@ -1688,8 +1688,7 @@ int small_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
     ch[c].max_iter = max_iter[c];
     xout[c] = X + off[c];
   }
-  TH2D_HIP(hipMemcpyAsync(dch, ch.data(), sizeof(SmallChain) * nch, hipMemcpyHostToDevice,
-                          ctx->stream));
+  TH2D_TRY(th2d::stage_h2d(ctx, dch, ch.data(), sizeof(SmallChain) * nch));
   if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k0, ctx->stream));
   if (maxV <= 256)
     hipLaunchKernelGGL((k_cg_small<256, 1, true>), dim3(nch), dim3(256), 0, ctx->stream, dch, dres);
@@ -1752,8 +1751,7 @@ int small_full_prepare(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, doub
     ch[c].tol_abs_rr = p->tol_abs_rr;
     ch[c].max_iter = p->max_iter;
   }
-  TH2D_HIP(hipMemcpyAsync(*dch, ch.data(), sizeof(SmallFull) * nch, hipMemcpyHostToDevice,
-                          ctx->stream));
+  TH2D_TRY(th2d::stage_h2d(ctx, *dch, ch.data(), sizeof(SmallFull) * nch));
   return TH2D_OK;
 }
 
@@ -2217,6 +2215,7 @@ int resident_solve_s(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   TH2D_HIP(hipMemcpyAsync((char *)ctx->redh.ptr + rb, ctl.err, 2 * sizeof(int),
                           hipMemcpyDeviceToHost, ctx->stream));
   TH2D_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->hstage_off = 0;  // the stream is idle: every staged copy has been read
   std::vector<CgResult> h(nch);
   memcpy(h.data(), ctx->redh.ptr, rb);
   int flags[2] = {0, 0};

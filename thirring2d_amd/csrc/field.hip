@@ -2,6 +2,8 @@
 // reductions and the device random number generator of libth2d (gfx950).
 #include "th2d_internal.h"
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstdlib>
 
@@ -79,6 +81,7 @@ extern "C" int th2d_ctx_destroy(th2d_ctx *c)
   buf_release(c->cgmix);
   buf_release(c->red);
   buf_release(c->redh, true);
+  buf_release(c->hstage, true);
   buf_release(c->hmc);
   buf_release(c->force);
   buf_release(c->bag);
@@ -462,7 +465,25 @@ int fetch_doubles(th2d_ctx *c, double *host, const double *dev, int n)
   TH2D_TRY(th2d_host_reserve(c->redh, sizeof(double) * n));
   TH2D_HIP(hipMemcpyAsync(c->redh.ptr, dev, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   TH2D_HIP(hipStreamSynchronize(c->stream));
+  c->hstage_off = 0;  // the stream is idle: every staged copy has been read
   memcpy(host, c->redh.ptr, sizeof(double) * n);
+  return TH2D_OK;
+}
+
+int stage_h2d(th2d_ctx *c, void *dst, const void *src, size_t bytes)
+{
+  const size_t need = (bytes + 255) & ~(size_t)255;
+  if (c->hstage_off + need > c->hstage.bytes) {
+    // the ring is reused from its start once every copy out of it is done
+    TH2D_HIP(hipStreamSynchronize(c->stream));
+    c->hstage_off = 0;
+    if (need > c->hstage.bytes)
+      TH2D_TRY(th2d_host_reserve(c->hstage, std::max(need, (size_t)1 << 18)));
+  }
+  char *p = (char *)c->hstage.ptr + c->hstage_off;
+  memcpy(p, src, bytes);
+  TH2D_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, c->stream));
+  c->hstage_off += need;
   return TH2D_OK;
 }
 

@@ -53,6 +53,8 @@ struct th2d_ctx {
   th2d_buf cg64, cg32, cg32b, cgmix, red, redh, hmc, force, bag, misc;  // cg32b: step-kernel
                                                                        // part of a split batch
   th2d_buf bpart, btab[4];  // batched site operations: partials, descriptor tables
+  th2d_buf hstage;          // pinned staging ring of small host-to-device copies
+  size_t hstage_off = 0;    // next free byte of the ring
   th2d_buf dsl_tab;                    // th2d_dslash_batched: its descriptor table,
   std::vector<unsigned char> dsl_key;  // uploaded again only when it changes
   // tile configuration of the persistent solver (0 = autotuned default)
@@ -364,6 +366,10 @@ int axpy_like(th2d_ctx *ctx, int op, long n, double a, double *y, const double *
               const double *z);
 int reduce_dot(th2d_ctx *ctx, long n_real, const double *a, const double *b, double *dev_out);
 int fetch_doubles(th2d_ctx *ctx, double *host, const double *dev, int n);
+// Host-to-device copy of a small block through the context's pinned
+// staging ring: the call returns once the copy is queued (a pageable
+// source would make it wait for the staging).
+int stage_h2d(th2d_ctx *ctx, void *dst, const void *src, size_t bytes);
 // Device address of the static scratch array of the allocation-free LU
 // inverse on the current device, looked up once per device (dense.hip).
 int noalloc_scratch_init(int device);
