@@ -1937,7 +1937,7 @@ int build_batch(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T>
   auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
   size_t part = al(sizeof(double) * 4 * (size_t)ntiles);
   size_t stb = al(sizeof(CgState) * 2 * (size_t)nch);
-  size_t rsb = al(sizeof(CgResult) * (size_t)nch);
+  size_t rsb = al(sizeof(CgResult) * (size_t)nch + 64);  // + the resident solver's flags
   size_t chb = al(sizeof(CgChain) * (size_t)nch);
   size_t tlb = al(sizeof(CgTile) * (size_t)ntiles);
   size_t exb = al(extra);
@@ -2016,9 +2016,10 @@ int finish_solve(th2d_ctx *ctx, int nch, const Geo *G, const CgBatch<T> &a,
   return TH2D_OK;
 }
 
-__global__ void k_zero_u32(unsigned *p, int n)
+__global__ void k_zero_u32(unsigned *p, int n, unsigned *q, int nq)
 {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) q[i] = 0u;
 }
 
 // Regions of a lattice, and of a batch, for regions of tt rows.
@@ -2156,8 +2157,8 @@ int resident_solve_s(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   if (groups.size() > 1 && 8.5 * (double)groups.size() > 10.0 + (double)sites * 120.0 / 4.5e6)
     return TH2D_ELAUNCH;  // (us per iteration) the step kernel is faster
   // control block: the workgroup table, per chain an iteration counter and a
-  // set-up counter (a cache line each), the error flags, the XCD table, the
-  // partials and the publication planes
+  // set-up counter (a cache line each), a spare line, the XCD table, the
+  // partials and the publication planes (the error flags follow the results)
   int nparts = 0;
   for (int c = 0; c < nch; c++)
     nparts += ((int)res_chain_regions(G[c], S::TT) + 7) & ~7;
@@ -2178,9 +2179,10 @@ int resident_solve_s(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   ctl.wg_tile = (const int *)extra;
   ctl.bar = (unsigned *)extra + ntab;
   ctl.bar0 = ctl.bar + 32 * nch;
-  ctl.err = (int *)(ctl.bar0 + 32 * nch);
+  // the two flags follow the results, so that one copy brings both back
+  ctl.err = (int *)((char *)a.res + sizeof(CgResult) * nch);
   ctl.misplaced = ctl.err + 1;
-  ctl.xcc = ctl.err + 32;
+  ctl.xcc = (int *)(ctl.bar0 + 32 * nch) + 32;
   ctl.part = (double *)(ctl.xcc + nxcc);  // 128-B aligned: every count above is a multiple of 32
   ctl.pub = (char *)ctl.part + 64 * NW * (size_t)nparts;  // 512-B aligned: nparts is a multiple of 8
   ctl.nparts = nparts;
@@ -2191,7 +2193,7 @@ int resident_solve_s(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   ctl.spin = ctx->res_spin_limit < 0 ? RES_SPIN_LIMIT : (unsigned)ctx->res_spin_limit;
   if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k0, ctx->stream));
   hipLaunchKernelGGL(k_zero_u32, dim3(std::min<int>(64, (int)(nzero + 255) / 256)), dim3(256), 0,
-                     ctx->stream, ctl.bar, (int)nzero);
+                     ctx->stream, ctl.bar, (int)nzero, (unsigned *)ctl.err, 2);
   TH2D_HIP(hipGetLastError());
   bool mu0 = true, clip = false;
   for (int c = 0; c < nch; c++) {
@@ -2208,12 +2210,11 @@ int resident_solve_s(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
     void *args[] = {(void *)&a, (void *)&ctl};
     TH2D_HIP(hipLaunchCooperativeKernel(kern, dim3(g.grid), dim3(S::NT), args, 0, ctx->stream));
   }
-  // results and the two flags into one pinned block, then one wait
+  // the results and the two flags after them into one pinned block, then one wait
   const size_t rb = sizeof(CgResult) * nch;
   TH2D_TRY(th2d_host_reserve(ctx->redh, rb + 64));
-  TH2D_HIP(hipMemcpyAsync(ctx->redh.ptr, a.res, rb, hipMemcpyDeviceToHost, ctx->stream));
-  TH2D_HIP(hipMemcpyAsync((char *)ctx->redh.ptr + rb, ctl.err, 2 * sizeof(int),
-                          hipMemcpyDeviceToHost, ctx->stream));
+  TH2D_HIP(hipMemcpyAsync(ctx->redh.ptr, a.res, rb + 2 * sizeof(int), hipMemcpyDeviceToHost,
+                          ctx->stream));
   TH2D_HIP(hipStreamSynchronize(ctx->stream));
   ctx->hstage_off = 0;  // the stream is idle: every staged copy has been read
   std::vector<CgResult> h(nch);
