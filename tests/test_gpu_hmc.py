@@ -496,9 +496,10 @@ def test_cg_resident_wait_timeout_falls_back(hmc, oracle):
     """A bounded wait of the resident solver that gives up (forced from the
     host with a spin limit of 0 polls: every workgroup still exits normally)
     makes the solve fall back to the step kernel from the same right-hand
-    sides: the solve and a batched trajectory both finish with every chain
-    at its true residual, and the fallback is counted.  With the default
-    limit the resident kernel runs again."""
+    sides (a pass whose resident solve was left in flight takes a zero
+    correction and is repeated): the solve and a batched trajectory both
+    finish with every chain at its true residual, and the fallbacks are
+    counted.  With the default limit the resident kernel runs again."""
     import ctypes as C
     from thirring2d_amd import _native as N
     setups = [_setup(hmc, oracle, 128, 128, 900 + k, m=0.1) for k in range(4)]

@@ -72,12 +72,14 @@ __global__ void __launch_bounds__(BT) k_bop(const BOp *__restrict__ ops, int nch
     const double2 a = ((const double2 *)o.d1)[i], b = ((const double2 *)o.d2)[i];
     ((double2 *)o.d0)[i] = make_double2(a.x - b.x, a.y - b.y);
   } else if (OP == BOP_TO_F32) {
+    const double s0 = o.sp ? 1.0 / sqrt(*o.sp) : o.s0;
     const double2 a = ((const double2 *)o.d1)[i];
-    ((float2 *)o.f0)[i] = make_float2((float)(a.x * o.s0), (float)(a.y * o.s0));
+    ((float2 *)o.f0)[i] = make_float2((float)(a.x * s0), (float)(a.y * s0));
   } else if (OP == BOP_ADD_F32) {
+    const double s0 = o.sp ? sqrt(*o.sp) : o.s0;
     double2 y = ((double2 *)o.d0)[i];
     const float2 x = ((const float2 *)o.f0)[i];
-    ((double2 *)o.d0)[i] = make_double2(y.x + o.s0 * (double)x.x, y.y + o.s0 * (double)x.y);
+    ((double2 *)o.d0)[i] = make_double2(y.x + s0 * (double)x.x, y.y + s0 * (double)x.y);
   } else if (OP == BOP_COPY) {
     o.d0[i] = o.d1[i];
     o.d0[V + i] = o.d1[V + i];
@@ -123,9 +125,22 @@ long bop_finish(std::vector<BOp> &ops)
 
 int bop_upload(th2d_ctx *c, th2d_buf &dev, const std::vector<BOp> &ops)
 {
+  for (int k = 0; k < 4; k++)
+    if (&dev == &c->btab[k]) c->btab_key[k].clear();  // its contents change
   const size_t bytes = sizeof(BOp) * ops.size();
   TH2D_TRY(th2d_buf_reserve(dev, bytes));
   return stage_h2d(c, dev.ptr, ops.data(), bytes);
+}
+
+int bop_upload_key(th2d_ctx *c, int slot, const std::vector<BOp> &ops)
+{
+  const size_t bytes = sizeof(BOp) * ops.size();
+  const unsigned char *raw = (const unsigned char *)ops.data();
+  std::vector<unsigned char> &key = c->btab_key[slot];
+  if (key.size() == bytes && memcmp(key.data(), raw, bytes) == 0) return TH2D_OK;
+  TH2D_TRY(bop_upload(c, c->btab[slot], ops));
+  key.assign(raw, raw + bytes);
+  return TH2D_OK;
 }
 
 int bop_launch(th2d_ctx *c, int op, const BOp *dev, int nch, long nblk)

@@ -878,17 +878,22 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
   // the batch with the step kernel)
   if (tid == 0)
     __hip_atomic_store(ctl.xcc + tau, xcc_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (!agent_sync(ctl.bar0 + 32 * tl.chain, (unsigned)ntc, ctl.err, red + 36, ctl.spin)) return;
-  if (tid == 0) {
-    const int mine = __hip_atomic_load(ctl.xcc + tau, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    bool same = true;
-    for (int k = ch.tbeg; k < ch.tend; k++)
-      same &= __hip_atomic_load(ctl.xcc + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == mine;
-    red[37] = same ? 1.0 : 0.0;
-    if (!same) __hip_atomic_store(ctl.misplaced, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // A solve that does not run or ends on a wait that timed out stores x = 0
+  // (with the error flag set): a zero correction for a caller that has
+  // queued its update of the solution before reading the flags.
+  bool failed = !agent_sync(ctl.bar0 + 32 * tl.chain, (unsigned)ntc, ctl.err, red + 36, ctl.spin);
+  if (!failed) {
+    if (tid == 0) {
+      const int mine = __hip_atomic_load(ctl.xcc + tau, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      bool same = true;
+      for (int k = ch.tbeg; k < ch.tend; k++)
+        same &= __hip_atomic_load(ctl.xcc + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == mine;
+      red[37] = same ? 1.0 : 0.0;
+      if (!same) __hip_atomic_store(ctl.misplaced, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    failed = red[37] == 0.0;  // the same decision in every region of the chain
   }
-  __syncthreads();
-  if (red[37] == 0.0) return;  // the same decision in every region of the chain
 
   const bool cont = ch.cscale > 0.0;
   double gam_old = cont ? ch.gam0 * ch.cscale * ch.cscale : 1.0;
@@ -1080,8 +1085,10 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     }
     slot = ns;
     // 1. every region of the chain has finished iteration j
-    if (!M.sync(o_bar, (unsigned)(j + 2) * (unsigned)ntc, ctl.err, red + 36, ctl.zero, ctl.spin))
+    if (!M.sync(o_bar, (unsigned)(j + 2) * (unsigned)ntc, ctl.err, red + 36, ctl.zero, ctl.spin)) {
+      failed = true;
       return false;
+    }
     // 2. the chain's partials of iteration j+1, summed by every wave in the
     // same order (a butterfly leaves the total in every lane), so that all
     // waves of all regions of the chain hold bitwise the same gam, del; then
@@ -1157,16 +1164,16 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     tb = (T)beta;
     return true;
   };
-  if (iteration(std::true_type{}, -1))
+  if (!failed && iteration(std::true_type{}, -1))
     for (int j = 0; iteration(std::false_type{}, j); j++) {
     }
-  // x of the owned sites, and p, s for a reliable update
+  // x of the owned sites (0 after a failure), and p, s for a reliable update
   V2 *xo = a.x + ch.voff;
 #pragma unroll
   for (int q = 0; q < NO; q++)
     if (!CLIP || ((inl >> q) & 1u)) {
       const unsigned oi = oi_of(q);
-      *at(xo, oi) = x[q];
+      *at(xo, oi) = failed ? zero : x[q];
       if (ch.cp) {
         *at((V2 *)ch.cp, oi) = p[q];
         *at((V2 *)ch.cs, oi) = s[q];
@@ -2210,8 +2217,26 @@ int resident_solve_s(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
     void *args[] = {(void *)&a, (void *)&ctl};
     TH2D_HIP(hipLaunchCooperativeKernel(kern, dim3(g.grid), dim3(S::NT), args, 0, ctx->stream));
   }
-  // the results and the two flags after them into one pinned block, then one wait
   const size_t rb = sizeof(CgResult) * nch;
+  if (ctx->cg_defer == 1) {
+    // left in flight: the results and flags follow into `resh`, and
+    // resident_collect reads them after the caller's next wait
+    if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k1, ctx->stream));
+    TH2D_TRY(th2d_host_reserve(ctx->resh, rb + 64));
+    TH2D_HIP(hipMemcpyAsync(ctx->resh.ptr, a.res, rb + 2 * sizeof(int), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    for (int c = 0; c < nch; c++) {
+      xout[c] = a.x + ch[c].voff;
+      res[c] = InnerResult{0, 0.0, TH2D_OK, 0.0, 0.0, 1};
+    }
+    ctx->cg_defer = 2;
+    ctx->res_pend_nch = nch;
+    ctx->res_pend_tt = S::TT;
+    ctx->res_pend_tx = S::TX;
+    ctx->res_pend_launches = (long)groups.size();
+    return TH2D_OK;
+  }
+  // the results and the two flags after them into one pinned block, then one wait
   TH2D_TRY(th2d_host_reserve(ctx->redh, rb + 64));
   TH2D_HIP(hipMemcpyAsync(ctx->redh.ptr, a.res, rb + 2 * sizeof(int), hipMemcpyDeviceToHost,
                           ctx->stream));
@@ -2251,6 +2276,39 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
     return resident_solve_s<RShapeH>(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, xout, res,
                                      carry);
   return resident_solve_s<RShape>(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, xout, res, carry);
+}
+
+// Results of the resident solve left in flight (cg_defer 2), once the
+// caller's stream has been waited for: TH2D_ESYNC after a wait that timed
+// out, TH2D_ELAUNCH after a misplacement (x is 0 then), or the per-chain
+// results in `res`.
+int resident_collect(th2d_ctx *ctx, int nch, const Geo *G, InnerResult *res)
+{
+  if (ctx->cg_defer != 2 || ctx->res_pend_nch != nch) return TH2D_EARG;
+  ctx->cg_defer = 0;
+  const size_t rb = sizeof(CgResult) * nch;
+  std::vector<CgResult> h(nch);
+  memcpy(h.data(), ctx->resh.ptr, rb);
+  int flags[2] = {0, 0};
+  memcpy(flags, (char *)ctx->resh.ptr + rb, sizeof(flags));
+  if (flags[1]) return TH2D_ELAUNCH;
+  if (flags[0]) return TH2D_ESYNC;
+  ctx->cg_last_tile[0] = ctx->res_pend_tt;
+  ctx->cg_last_tile[1] = ctx->res_pend_tx;
+  ctx->cg_last_resident = 1;
+  if (ctx->ktiming) {
+    float ms = 0.f;
+    TH2D_HIP(hipEventElapsedTime(&ms, ctx->k0, ctx->k1));
+    ctx->ktime_ms += ms;
+    ctx->klaunches += ctx->res_pend_launches;
+    for (int c = 0; c < nch; c++) {  // algorithmic bytes as finish_solve counts them
+      ctx->kbytes += (double)h[c].iters * 2.0 * G[c].vh * 15.0 * sizeof(c2<float>);
+      ctx->kiters += h[c].iters;
+    }
+  }
+  for (int c = 0; c < nch; c++)
+    res[c] = InnerResult{h[c].iters, h[c].rr, h[c].status, h[c].gam_old, h[c].alpha_old, 1};
+  return TH2D_OK;
 }
 
 // Whether the resident solver takes a lattice: eligible, and no more
@@ -2324,10 +2382,14 @@ int inner_solve(th2d_ctx *ctx, th2d_buf &buf, int pass, int nch, const Geo *G,
       std::vector<int> ra, rb;
       for (int c = 0; c < nch; c++) (res_takes(G[c], cap) ? ra : rb).push_back(c);
       int rc = TH2D_ELAUNCH;
-      if (rb.empty())
+      if (rb.empty()) {
         rc = resident_solve(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, xout, res, carry);
-      else if (!ra.empty())
+      } else if (!ra.empty()) {
+        const int defer = ctx->cg_defer;  // a split batch is solved in place
+        ctx->cg_defer = 0;
         rc = split_solve(ctx, buf, pass, G, rhs, K, stop_rr, max_iter, xout, res, carry, ra, rb);
+        ctx->cg_defer = defer;
+      }
       // A wait that timed out (TH2D_ESYNC) left every workgroup exiting
       // normally and the right-hand sides untouched: the step kernel
       // re-solves the batch, as it does for a batch the resident kernel
@@ -2596,6 +2658,7 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
     rr[c] = bb[c];
     if (bb[c] == 0.0 || rr[c] <= target[c]) active[c] = 0;
   }
+  bool res_defer = true;  // resident inner solves left in flight (mixed)
   for (int outer = 0; outer < 64; outer++) {
     std::vector<int> idx;
     for (int c = 0; c < nch; c++)
@@ -2618,6 +2681,10 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
       tin[k].d1 = (double *)rhs;
       tin[k].s0 = 1.0 / nr;
       if (mixed) {
+        // the scale from the chain's |r|^2 on the device: the table stays
+        // the same from pass to pass (uploaded again only when it changes)
+        tin[k].s0 = 0.0;
+        tin[k].sp = dsc + c;
         tin[k].f0 = (float *)(R32 + voff[c]);
         stop[k] = std::max(delta * delta, target[c] / rr[c]);
       } else if (outer == 0) {
@@ -2629,7 +2696,8 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
     }
     const long nblk_in = th2d::bop_finish(tin);
     if (mixed || outer > 0) {
-      TH2D_TRY(th2d::bop_upload(ctx, ctx->btab[0], tin));
+      if (mixed) TH2D_TRY(th2d::bop_upload_key(ctx, 0, tin));
+      else TH2D_TRY(th2d::bop_upload(ctx, ctx->btab[0], tin));
       TH2D_TRY(th2d::bop_launch(ctx, mixed ? th2d::BOP_TO_F32 : th2d::BOP_SCALE,
                                 (const BOp *)ctx->btab[0].ptr, na, nblk_in));
     }
@@ -2637,6 +2705,15 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
     // r = b - M^dag M x with |r|^2 in FP64: three batched launches
     std::vector<BOp> tx(na), tm(na), tres(na);
     std::vector<char> was_cont(na, 0);
+    bool deferred = false;
+    auto note_carry = [&](int n, const InnerResult *q) {  // (kept only if the pass stays consistent)
+      for (int k = 0; k < n; k++) {
+        const int c = idx[k];
+        carry_ok[c] = q[k].resident && q[k].status != TH2D_EBREAKDOWN;
+        cgam[c] = q[k].gam;
+        calpha[c] = q[k].alpha;
+      }
+    };
     if (mixed) {
       std::vector<const float2 *> rh(na), kk(na);
       std::vector<float2 *> xo(na);
@@ -2657,19 +2734,22 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
         }
         nprev[c] = sqrt(rr[c]);
       }
-      TH2D_TRY(inner_solve<float>(ctx, ctx->cg32, outer, na, g2.data(), rh.data(), kk.data(),
-                                  stop.data(), mi.data(), xo.data(), ir.data(), -1, cy.data()));
-      for (int k = 0; k < na; k++) {  // (kept only if the pass stays consistent, below)
-        const int c = idx[k];
-        carry_ok[c] = ir[k].resident && ir[k].status != TH2D_EBREAKDOWN;
-        cgam[c] = ir[k].gam;
-        calpha[c] = ir[k].alpha;
-      }
+      // A resident solve may be left in flight (its x is then 0 if it
+      // fails): the FP64 update and residual below are queued behind it,
+      // and one wait reads the solver's results with |r|^2.
+      ctx->cg_defer = res_defer ? 1 : 0;
+      const int irc = inner_solve<float>(ctx, ctx->cg32, outer, na, g2.data(), rh.data(),
+                                         kk.data(), stop.data(), mi.data(), xo.data(), ir.data(),
+                                         -1, cy.data());
+      deferred = ctx->cg_defer == 2;
+      if (!deferred) ctx->cg_defer = 0;
+      TH2D_TRY(irc);
+      if (!deferred) note_carry(na, ir.data());
       for (int k = 0; k < na; k++) {
         tx[k] = op_of(idx[k]);
         tx[k].d0 = x[idx[k]];
         tx[k].f0 = (float *)xo[k];
-        tx[k].s0 = sqrt(rr[idx[k]]);
+        tx[k].sp = dsc + idx[k];  // |r| of the pass's right-hand side, before RESID replaces it
       }
     } else {
       std::vector<const double2 *> rh(na), kk(na);
@@ -2705,19 +2785,34 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
       tres[k].d2 = (double *)K[c];
       tres[k].d3 = tmp;
       tres[k].s1 = 1.0;
-      tres[k].out = dsc + k;
+      tres[k].out = dsc + c;
     }
     const long nblk = th2d::bop_finish(tx);
     for (auto *t : {&tm, &tres}) th2d::bop_finish(*t);
     std::vector<BOp> all(tx);
     for (auto *t : {&tm, &tres}) all.insert(all.end(), t->begin(), t->end());
-    TH2D_TRY(th2d::bop_upload(ctx, ctx->btab[1], all));
+    if (mixed) TH2D_TRY(th2d::bop_upload_key(ctx, 1, all));
+    else TH2D_TRY(th2d::bop_upload(ctx, ctx->btab[1], all));
     const BOp *dt = (const BOp *)ctx->btab[1].ptr;
     TH2D_TRY(th2d::bop_launch(ctx, mixed ? th2d::BOP_ADD_F32 : th2d::BOP_AXPY, dt, na, nblk));
     TH2D_TRY(th2d::bop_launch(ctx, th2d::BOP_DSLASH, dt + na, na, nblk));
     TH2D_TRY(th2d::bop_launch(ctx, th2d::BOP_RESID, dt + 2 * na, na, nblk));
-    std::vector<double> rn(na);
-    TH2D_TRY(th2d::fetch_doubles(ctx, rn.data(), dsc, na));
+    std::vector<double> rnc(nch), rn(na);  // |r|^2 per chain (slot c)
+    TH2D_TRY(th2d::fetch_doubles(ctx, rnc.data(), dsc, nch));
+    for (int k = 0; k < na; k++) rn[k] = rnc[idx[k]];
+    if (deferred) {
+      const int crc = resident_collect(ctx, na, g2.data(), ir.data());
+      if (crc == TH2D_ESYNC || crc == TH2D_ELAUNCH) {
+        // x took a zero correction: the pass is repeated, in place (with the
+        // step kernel as the fallback), and counts nothing
+        if (crc == TH2D_ESYNC) ctx->cg_sync_fallbacks++;
+        res_defer = false;
+        for (int k = 0; k < na; k++) carry_ok[idx[k]] = 0;
+        continue;
+      }
+      TH2D_TRY(crc);
+      note_carry(na, ir.data());
+    }
     for (int k = 0; k < na; k++) {
       int c = idx[k];
       inf[c].iterations += ir[k].iters;

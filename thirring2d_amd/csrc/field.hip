@@ -82,6 +82,7 @@ extern "C" int th2d_ctx_destroy(th2d_ctx *c)
   buf_release(c->red);
   buf_release(c->redh, true);
   buf_release(c->hstage, true);
+  buf_release(c->resh, true);
   buf_release(c->hmc);
   buf_release(c->force);
   buf_release(c->bag);

@@ -53,6 +53,7 @@ struct th2d_ctx {
   th2d_buf cg64, cg32, cg32b, cgmix, red, redh, hmc, force, bag, misc;  // cg32b: step-kernel
                                                                        // part of a split batch
   th2d_buf bpart, btab[4];  // batched site operations: partials, descriptor tables
+  std::vector<unsigned char> btab_key[4];  // bytes last uploaded to btab[i] by bop_upload_key
   th2d_buf hstage;          // pinned staging ring of small host-to-device copies
   size_t hstage_off = 0;    // next free byte of the ring
   th2d_buf dsl_tab;                    // th2d_dslash_batched: its descriptor table,
@@ -78,6 +79,13 @@ struct th2d_ctx {
   long res_spin_limit = -1;       // polls of one resident-kernel wait (-1: the default)
   int res_rows = 0;               // resident regions of 64 or 32 rows (0: chosen per batch)
   long cg_sync_fallbacks = 0;     // resident solves that timed out and were re-solved
+  // A resident inner solve left in flight for the mixed solver's outer loop
+  // (cg.hip): 1 asks for it, 2 says one was launched; its results and flags
+  // are copied into `resh` behind it.
+  int cg_defer = 0;
+  int res_pend_nch = 0, res_pend_tt = 0, res_pend_tx = 0;
+  long res_pend_launches = 0;
+  th2d_buf resh;
   std::map<unsigned long long, int> cg_tuned[2];  // signature -> shape (FP32, FP64)
   // step-kernel launches captured as graphs (cg.hip), keyed by the launch
   // arguments; cg_graph 0 (default) launches eagerly
@@ -319,6 +327,7 @@ struct BOp {
   double *d0, *d1, *d2, *d3, *d4;
   float *f0;
   double *out;                 // reduction result (device)
+  const double *sp;            // TO_F32, ADD_F32: s0 = 1 / sqrt(*sp), sqrt(*sp) (device)
 };
 
 __device__ __forceinline__ int bop_chain(const BOp *ops, int nch, long blk)
@@ -394,8 +403,8 @@ enum {
   BOP_FORCE,       // mom(d0) -= s0 F; K = d2, chi = d1, Mchi = d3, st = d4 (may be NULL); s1 = Nf/g
   BOP_AXPY,        // d0 += s0 d1
   BOP_SUB,         // d0 = d1 - d2
-  BOP_TO_F32,      // f0 = (float)(s0 d1)
-  BOP_ADD_F32,     // d0 += s0 f0
+  BOP_TO_F32,      // f0 = (float)(s0 d1); s0 = 1 / sqrt(*sp) when sp is set
+  BOP_ADD_F32,     // d0 += s0 f0; s0 = sqrt(*sp) when sp is set
   BOP_COPY,        // d0 = d1 (both link fields: 2 reals per direction)
   BOP_SCALE,       // d0 = s0 d1
   BOP_DOT,         // out = s1 * sum_sites Re(d0 . d1)  (two-pass, fixed order)
@@ -405,5 +414,7 @@ enum {
 // Fills blk0 and returns the block count.
 long bop_finish(std::vector<BOp> &ops);
 int bop_upload(th2d_ctx *ctx, th2d_buf &dev, const std::vector<BOp> &ops);
+// The same into btab[slot], skipped when the bytes equal the last upload's.
+int bop_upload_key(th2d_ctx *ctx, int slot, const std::vector<BOp> &ops);
 int bop_launch(th2d_ctx *ctx, int op, const BOp *dev, int nch, long nblk);
 }  // namespace th2d
