@@ -100,7 +100,10 @@ __global__ void __launch_bounds__(BT) k_bop_finish(const BOp *__restrict__ ops, 
   double acc = 0.0;
   for (long b = o.blk0 + threadIdx.x; b < end; b += BT) acc += part[b];
   const double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) *o.out = s * o.s1;
+  if (threadIdx.x == 0) {
+    *o.out = s * o.s1;
+    if (o.d4) *o.d4 = s * o.s1;  // a second copy (RESID; d4 is unused by the other reductions)
+  }
 }
 
 template <int OP>

@@ -1944,7 +1944,9 @@ int build_batch(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G, const c2<T>
   auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
   size_t part = al(sizeof(double) * 4 * (size_t)ntiles);
   size_t stb = al(sizeof(CgState) * 2 * (size_t)nch);
-  size_t rsb = al(sizeof(CgResult) * (size_t)nch + 64);  // + the resident solver's flags
+  // + the resident solver's flags, and |r|^2 of each chain for a caller
+  // that reads them with the results (resident_collect)
+  size_t rsb = al(sizeof(CgResult) * (size_t)nch + 64 + sizeof(double) * (size_t)nch);
   size_t chb = al(sizeof(CgChain) * (size_t)nch);
   size_t tlb = al(sizeof(CgTile) * (size_t)ntiles);
   size_t exb = al(extra);
@@ -2219,12 +2221,10 @@ int resident_solve_s(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   }
   const size_t rb = sizeof(CgResult) * nch;
   if (ctx->cg_defer == 1) {
-    // left in flight: the results and flags follow into `resh`, and
-    // resident_collect reads them after the caller's next wait
+    // left in flight: resident_collect copies the results, the flags and the
+    // caller's |r|^2 (written behind them) back after the caller's work
     if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k1, ctx->stream));
-    TH2D_TRY(th2d_host_reserve(ctx->resh, rb + 64));
-    TH2D_HIP(hipMemcpyAsync(ctx->resh.ptr, a.res, rb + 2 * sizeof(int), hipMemcpyDeviceToHost,
-                            ctx->stream));
+    ctx->res_pend_dev = a.res;
     for (int c = 0; c < nch; c++) {
       xout[c] = a.x + ch[c].voff;
       res[c] = InnerResult{0, 0.0, TH2D_OK, 0.0, 0.0, 1};
@@ -2278,15 +2278,30 @@ int resident_solve(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   return resident_solve_s<RShape>(ctx, buf, nch, G, rhs, K, stop_rr, max_iter, xout, res, carry);
 }
 
-// Results of the resident solve left in flight (cg_defer 2), once the
-// caller's stream has been waited for: TH2D_ESYNC after a wait that timed
-// out, TH2D_ELAUNCH after a misplacement (x is 0 then), or the per-chain
-// results in `res`.
-int resident_collect(th2d_ctx *ctx, int nch, const Geo *G, InnerResult *res)
+// Where the caller of a resident solve left in flight writes |r|^2 of its
+// chain k (slot k), so that resident_collect brings it back with the results.
+double *resident_rr_slots(th2d_ctx *ctx)
+{
+  if (ctx->cg_defer != 2) return nullptr;
+  return (double *)((char *)ctx->res_pend_dev + sizeof(CgResult) * ctx->res_pend_nch + 64);
+}
+
+// Results of the resident solve left in flight (cg_defer 2), after the
+// caller's work on the stream: one copy back and one wait.  TH2D_ESYNC
+// after a wait that timed out, TH2D_ELAUNCH after a misplacement (x is 0
+// then), or the per-chain results in `res`; `rr` (if set) receives the
+// caller's |r|^2 slots either way.
+int resident_collect(th2d_ctx *ctx, int nch, const Geo *G, InnerResult *res, double *rr)
 {
   if (ctx->cg_defer != 2 || ctx->res_pend_nch != nch) return TH2D_EARG;
   ctx->cg_defer = 0;
-  const size_t rb = sizeof(CgResult) * nch;
+  const size_t rb = sizeof(CgResult) * nch, bytes = rb + 64 + sizeof(double) * nch;
+  TH2D_TRY(th2d_host_reserve(ctx->resh, bytes));
+  TH2D_HIP(hipMemcpyAsync(ctx->resh.ptr, ctx->res_pend_dev, bytes, hipMemcpyDeviceToHost,
+                          ctx->stream));
+  TH2D_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->hstage_off = 0;  // the stream is idle: every staged copy has been read
+  if (rr) memcpy(rr, (char *)ctx->resh.ptr + rb + 64, sizeof(double) * nch);
   std::vector<CgResult> h(nch);
   memcpy(h.data(), ctx->resh.ptr, rb);
   int flags[2] = {0, 0};
@@ -2786,6 +2801,7 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
       tres[k].d3 = tmp;
       tres[k].s1 = 1.0;
       tres[k].out = dsc + c;
+      if (deferred) tres[k].d4 = resident_rr_slots(ctx) + k;  // also behind the results
     }
     const long nblk = th2d::bop_finish(tx);
     for (auto *t : {&tm, &tres}) th2d::bop_finish(*t);
@@ -2797,11 +2813,16 @@ extern "C" int th2d_cg_mdagm_batched(th2d_ctx *ctx, int nch, const th2d_geom *ge
     TH2D_TRY(th2d::bop_launch(ctx, mixed ? th2d::BOP_ADD_F32 : th2d::BOP_AXPY, dt, na, nblk));
     TH2D_TRY(th2d::bop_launch(ctx, th2d::BOP_DSLASH, dt + na, na, nblk));
     TH2D_TRY(th2d::bop_launch(ctx, th2d::BOP_RESID, dt + 2 * na, na, nblk));
-    std::vector<double> rnc(nch), rn(na);  // |r|^2 per chain (slot c)
-    TH2D_TRY(th2d::fetch_doubles(ctx, rnc.data(), dsc, nch));
-    for (int k = 0; k < na; k++) rn[k] = rnc[idx[k]];
+    std::vector<double> rn(na);
+    int crc = TH2D_OK;
+    if (deferred) {  // one copy back and one wait for the results and |r|^2
+      crc = resident_collect(ctx, na, g2.data(), ir.data(), rn.data());
+    } else {
+      std::vector<double> rnc(nch);  // |r|^2 per chain (slot c)
+      TH2D_TRY(th2d::fetch_doubles(ctx, rnc.data(), dsc, nch));
+      for (int k = 0; k < na; k++) rn[k] = rnc[idx[k]];
+    }
     if (deferred) {
-      const int crc = resident_collect(ctx, na, g2.data(), ir.data());
       if (crc == TH2D_ESYNC || crc == TH2D_ELAUNCH) {
         // x took a zero correction: the pass is repeated, in place (with the
         // step kernel as the fallback), and counts nothing

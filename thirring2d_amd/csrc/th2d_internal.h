@@ -85,6 +85,7 @@ struct th2d_ctx {
   int cg_defer = 0;
   int res_pend_nch = 0, res_pend_tt = 0, res_pend_tx = 0;
   long res_pend_launches = 0;
+  void *res_pend_dev = nullptr;  // its result block (device)
   th2d_buf resh;
   std::map<unsigned long long, int> cg_tuned[2];  // signature -> shape (FP32, FP64)
   // step-kernel launches captured as graphs (cg.hip), keyed by the launch
@@ -409,7 +410,7 @@ enum {
   BOP_SCALE,       // d0 = s0 d1
   BOP_DOT,         // out = s1 * sum_sites Re(d0 . d1)  (two-pass, fixed order)
   BOP_COSGAP,      // out = s1 * sum_links (1 - cos d0)
-  BOP_RESID,       // d0 = d1 - M^dag d3 (K = d2); out = s1 * sum_sites |d0|^2
+  BOP_RESID,       // d0 = d1 - M^dag d3 (K = d2); out (and d4 if set) = s1 * sum_sites |d0|^2
 };
 // Fills blk0 and returns the block count.
 long bop_finish(std::vector<BOp> &ops);
