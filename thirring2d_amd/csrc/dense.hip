@@ -229,7 +229,9 @@ __global__ void __launch_bounds__(256) k_lu_panel(T *base, BatchView bv, int *pi
 // column and the rank-1 update of the panel's later columns on registers.  Only the panel's columns are swapped
 // here; k_lu_swap_trsm applies the panel's interchanges to the other columns
 // afterwards, in order, so the factors equal those of whole-row swaps.
-constexpr int PNB = 16;   // register panel width
+// register panel width: 16 columns in FP64, 32 in FP32 (the same bytes of
+// registers per thread; half the panels, interchange and update launches)
+template <typename T> constexpr int pnb = sizeof(T) == 4 ? 32 : 16;
 constexpr int PT = 1024;  // threads of the register panel
 constexpr int PR = 3;     // panel rows per thread
 constexpr int PCH = 256;  // rows per LDS staging chunk of the panel's load and store
@@ -289,7 +291,7 @@ template <typename T, int NT>
 __global__ void __launch_bounds__(NT) k_lu_panel_reg(T *base, BatchView bv, int *pivb, int *info,
                                                      int k)
 {
-  constexpr int CH = PCH;
+  constexpr int CH = PCH, PNB = pnb<T>;
   // double-buffered by column parity: a buffer is rewritten two columns
   // later, after every thread has passed the barrier between
   __shared__ double wbest[2][NT / 64];
@@ -445,6 +447,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_lu_swap_trsm(T *base, BatchView bv, const int *pivb,
                                                       int k, int c1, int sw0, int sw1, int cbase)
 {
+  constexpr int PNB = pnb<T>;
   __shared__ T sL[PNB * PNB];
   const int b = blockIdx.y;
   const int n = bv.n[b];
@@ -827,6 +830,7 @@ int lu_run(th2d_ctx *c, const th2d_batch *bt, T *A, int *piv, int *info)
   const bool mf = use_mfma();
   const int N = bt->max_n, cnt = bt->count;
   constexpr int END = 1 << 30;  // a range bound past every order
+  constexpr int PNB = pnb<T>;
   hipStream_t ps = nullptr, us = nullptr;
   if (bt->max_n <= PR * PT && g_lu_reg_panel && g_lu_lookahead && N > LOB &&
       lu_streams(c, &ps, &us)) {
