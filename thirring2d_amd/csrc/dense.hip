@@ -82,7 +82,24 @@ __device__ void gemm_tile(const T *A, long lda, const T *B, long ldb, T *C, long
   // registers while the current chunk is multiplied out of LDS
   constexpr int EA = BM * BK / 256, EB = BK * BN / 256;
   T ra[EA], rb[EB];
+  // an FP64 tile inside the matrix loads its chunks without bounds checks
+  // (a uniform branch per chunk), so the loads issue back to back; in FP32
+  // the second path costs registers, and so occupancy
+  const bool inner = sizeof(T) == 8 && i0 + BM <= M && j0 + BN <= N;
   auto fetch = [&](int k0) {
+    if (inner && k0 + BK <= K) {
+#pragma unroll
+      for (int q = 0; q < EA; q++) {
+        const int e = tid + q * 256, r = e / BK, c = e - r * BK;
+        ra[q] = A[(i0 + r) * lda + k0 + c];
+      }
+#pragma unroll
+      for (int q = 0; q < EB; q++) {
+        const int e = tid + q * 256, r = e / BN, c = e - r * BN;
+        rb[q] = B[(k0 + r) * ldb + j0 + c];
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < EA; q++) {
       const int e = tid + q * 256, r = e / BK, c = e - r * BK;
@@ -137,7 +154,7 @@ __device__ void gemm_tile(const T *A, long lda, const T *B, long ldb, T *C, long
       for (int r = 0; r < 4; r++) {
         int gi = i0 + wr + a * 16 + Mfma<T>::row(lane, r);
         int gj = j0 + wc + b * 16 + (lane & 15);
-        if (gi < M && gj < N) {
+        if (inner || (gi < M && gj < N)) {
           T v = USE_MFMA ? acc[a][b][r] : vacc[a * 2 + b][r];
           C[gi * ldc + gj] += alpha * v;
         }
@@ -555,9 +572,11 @@ __global__ void __launch_bounds__(256) k_lu_trsm(T *base, BatchView bv, int k, i
 // A[r0:r1, c0:c1] -= A[r0:r1, kk:kk+K] A[kk:kk+K, c0:c1], every range
 // clipped to the matrix's order: the trailing and in-block updates of the
 // LU (L21 U12).
+// (FP64 on the matrix cores: at most 128 VGPRs, four waves per SIMD)
 template <typename T, bool MF>
-__global__ void __launch_bounds__(256) k_lu_update(T *base, BatchView bv, int kk, int K, int r0,
-                                                   int r1, int c0, int c1)
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && MF ? 4 : 1)))
+    k_lu_update(T *base, BatchView bv, int kk, int K, int r0, int r1, int c0, int c1)
 {
   __shared__ T sA[BM * (BK + 1)];
   __shared__ T sB[BK * (BN + 1)];
@@ -682,7 +701,9 @@ __global__ void __launch_bounds__(256) k_trsm_rows(const T *LU, T *X, BatchView 
 // Forward: X[k+jb:, :] -= L[k+jb:, k:k+jb] X[k:k+jb, :]
 // Backward: X[0:k, :] -= U[0:k, k:k+jb] X[k:k+jb, :]
 template <typename T, bool MF, bool BACK>
-__global__ void __launch_bounds__(256) k_inv_update(const T *LU, T *X, BatchView bv, int k)
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && MF ? 4 : 1)))
+    k_inv_update(const T *LU, T *X, BatchView bv, int k)
 {
   __shared__ T sA[BM * (BK + 1)];
   __shared__ T sB[BK * (BN + 1)];
@@ -703,7 +724,9 @@ __global__ void __launch_bounds__(256) k_inv_update(const T *LU, T *X, BatchView
 
 // generic batched GEMM: C += alpha A B (jobs in device memory)
 template <typename T, bool MF>
-__global__ void __launch_bounds__(256) k_gemm_jobs(T *base, const th2d_gemm_job *jobs, double alpha)
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && MF ? 4 : 1)))
+    k_gemm_jobs(T *base, const th2d_gemm_job *jobs, double alpha)
 {
   __shared__ T sA[BM * (BK + 1)];
   __shared__ T sB[BK * (BN + 1)];
