@@ -251,7 +251,6 @@ __global__ void __launch_bounds__(256) k_lu_panel(T *base, BatchView bv, int *pi
 template <typename T> constexpr int pnb = sizeof(T) == 4 ? 32 : 16;
 constexpr int PT = 1024;  // threads of the register panel
 constexpr int PR = 3;     // panel rows per thread
-constexpr int PCH = 256;  // rows per LDS staging chunk of the panel's load and store
 constexpr int LOB = 128;  // outer block of the register-panel LU
 
 // DPP move of a double across lanes; lanes the move does not write keep v.
@@ -304,18 +303,40 @@ __device__ __forceinline__ int row16_min(int v)
   return __builtin_amdgcn_readlane(v, 0);
 }
 
+// One element of a matrix by a buffer access at a byte offset from its
+// first element.  An offset at or beyond the resource's size (the matrix's
+// n*n elements) reads 0 and drops the store.
+constexpr unsigned BUF_OOR = 0x80000000u;
+template <typename T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t rs, unsigned voff)
+{
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, 0, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 0, 0));
+}
+template <typename T>
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t rs, unsigned voff, T v)
+{
+  typedef unsigned u2v __attribute__((ext_vector_type(2)));
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, v), rs, voff, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, voff, 0, 0);
+}
+
 template <typename T, int NT>
 __global__ void __launch_bounds__(NT) k_lu_panel_reg(T *base, BatchView bv, int *pivb, int *info,
                                                      int k)
 {
-  constexpr int CH = PCH, PNB = pnb<T>;
+  constexpr int PNB = pnb<T>;
   // double-buffered by column parity: a buffer is rewritten two columns
   // later, after every thread has passed the barrier between
   __shared__ double wbest[2][NT / 64];
   __shared__ int widx[2][NT / 64];
   __shared__ T wrow[2][NT / 64][PNB];  // each wave's candidate row
   __shared__ T drow[2][PNB];      // the diagonal row before the exchange
-  __shared__ T stage[CH * (PNB + 1)];
+  __shared__ T stage[NT * (PNB + 1)];  // one slab of NT rows (at most 136 KiB)
   const int b = blockIdx.x;
   const int n = bv.n[b];
   if (k >= n) return;
@@ -327,25 +348,33 @@ __global__ void __launch_bounds__(NT) k_lu_panel_reg(T *base, BatchView bv, int 
   for (int q = 0; q < PR; q++)
 #pragma unroll
     for (int c = 0; c < PNB; c++) a[q][c] = (T)0;
-  // rows arrive through LDS in chunks of CH, read row-contiguously (16
-  // lanes per row) rather than one row per lane
+  // rows arrive through LDS in slabs of NT rows, one per q, read
+  // row-contiguously (PNB consecutive lanes per row) rather than one row per
+  // lane; a thread's loads of a slab are all issued before the first wait.
+  // Thread t moves column t % PNB of rows t / PNB + m * NT / PNB; elements
+  // outside the matrix get an out-of-range offset.
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(A, 0, (int)((long)n * n * sizeof(T)), 0x00020000);
+  const int sc = tid % PNB, sr = tid / PNB;
+  constexpr int SRS = NT / PNB;  // rows between a thread's elements
+  auto soff = [&](int sr, int sc, int r0, int m) {
+    const int i = k + r0 + sr + m * SRS;
+    return (sc < jb && i < n) ? (unsigned)(((long)i * n + k + sc) * sizeof(T)) : BUF_OOR;
+  };
 #pragma unroll
-  for (int q = 0; q < PR; q++)
+  for (int q = 0; q < PR; q++) {
+    const int r0 = q * NT;  // first panel row of the slab, from k
+    if (k + r0 >= n) break;
+    if (q) __syncthreads();  // the previous slab has been read
+    T v[PNB];
 #pragma unroll
-    for (int s = 0; s < NT / CH; s++) {
-      const int r0 = q * NT + s * CH;  // first panel row of the chunk, from k
-      if (k + r0 >= n) continue;
-      __syncthreads();
-      for (int e = tid; e < CH * PNB; e += NT) {
-        const int r = e / PNB, c = e - r * PNB, i = k + r0 + r;
-        stage[r * (PNB + 1) + c] = (i < n && c < jb) ? A[(long)i * n + k + c] : (T)0;
-      }
-      __syncthreads();
-      const int r = tid - s * CH;
-      if (r >= 0 && r < CH)
+    for (int m = 0; m < PNB; m++) v[m] = buf_ld<T>(rs, soff(sr, sc, r0, m));
 #pragma unroll
-        for (int c = 0; c < PNB; c++) a[q][c] = stage[r * (PNB + 1) + c];
-    }
+    for (int m = 0; m < PNB; m++) stage[(sr + m * SRS) * (PNB + 1) + sc] = v[m];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < PNB; c++) a[q][c] = stage[tid * (PNB + 1) + c];
+  }
 #pragma clang loop unroll(full)
   for (int j = 0; j < PNB; j++) {
     if (j >= jb) continue;  // (a full unroll keeps a[][] in registers)
@@ -436,23 +465,24 @@ __global__ void __launch_bounds__(NT) k_lu_panel_reg(T *base, BatchView bv, int 
       }
     }
   }
+  // the thread's row and column in a slab, recomputed here rather than
+  // kept in registers through the column loop
+  int t2 = tid;
+  __asm__ volatile("" : "+v"(t2));
+  const int sc2 = t2 % PNB, sr2 = t2 / PNB;
 #pragma unroll
-  for (int q = 0; q < PR; q++)
+  for (int q = 0; q < PR; q++) {
+    const int r0 = q * NT;
+    if (k + r0 >= n) break;
+    if (q) __syncthreads();  // the slab before has been stored (the column loop's
+                             // barriers separate the first from the loads' reads)
 #pragma unroll
-    for (int s = 0; s < NT / CH; s++) {
-      const int r0 = q * NT + s * CH;
-      if (k + r0 >= n) continue;
-      __syncthreads();
-      const int r = tid - s * CH;
-      if (r >= 0 && r < CH)
+    for (int c = 0; c < PNB; c++) stage[tid * (PNB + 1) + c] = a[q][c];
+    __syncthreads();
 #pragma unroll
-        for (int c = 0; c < PNB; c++) stage[r * (PNB + 1) + c] = a[q][c];
-      __syncthreads();
-      for (int e = tid; e < CH * PNB; e += NT) {
-        const int r2 = e / PNB, c = e - r2 * PNB, i = k + r0 + r2;
-        if (i < n && c < jb) A[(long)i * n + k + c] = stage[r2 * (PNB + 1) + c];
-      }
-    }
+    for (int m = 0; m < PNB; m++)
+      buf_st<T>(rs, soff(sr2, sc2, r0, m), stage[(sr2 + m * SRS) * (PNB + 1) + sc2]);
+  }
 }
 
 // The row interchanges of the panel at k (pivots k..k+jb-1, in order) on
