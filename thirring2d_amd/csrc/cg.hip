@@ -963,7 +963,8 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     // r(t +- 1) come from r[] and K_t(t-1) from the previous row, so an
     // element loads its two x neighbours and three links from LDS (the
     // rows above and below the wave's once).  Then the frame's first and
-    // last rows and columns, at most one element per thread.
+    // last rows and columns without the corners, at most one element per
+    // thread.
     {
       const int c = (wrow + 2) * W2 + (lane + 2);  // R2 index of 1-halo (wrow + 1, lane + 1)
       // r on the wave's rows is r[] (3a wrote the same values to R2); the
@@ -1005,13 +1006,16 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
         }
         __builtin_amdgcn_sched_barrier(0);
       }
-      for (int e = tid; e < 2 * W1 + 2 * TT; e += NT) {
+      // (3c reads no corner of the frame: 2 TX + 2 TT elements, one per
+      // thread of the first waves, one wave per SIMD at 64 x 64 with 512
+      // threads)
+      for (int e = tid; e < 2 * TX + 2 * TT; e += NT) {
         int ft, fx;
-        if (e < 2 * W1) {  // rows 0 and TT + 1, every column
-          ft = e < W1 ? 0 : TT + 1;
-          fx = e < W1 ? e : e - W1;
+        if (e < 2 * TX) {  // rows 0 and TT + 1, columns 1 .. TX
+          ft = e < TX ? 0 : TT + 1;
+          fx = 1 + (e < TX ? e : e - TX);
         } else {  // columns 0 and W1 - 1 of rows 1 .. TT
-          const int k = e - 2 * W1;
+          const int k = e - 2 * TX;
           ft = 1 + (k >> 1);
           fx = (k & 1) ? W1 - 1 : 0;
         }
