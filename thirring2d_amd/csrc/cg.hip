@@ -515,10 +515,11 @@ template <typename V> __device__ __forceinline__ const V *at(const V *base, unsi
   return (const V *)((const char *)base + off);
 }
 
-// Invalidates this CU's vector L1, so that the next loads read L2.
-__device__ __forceinline__ void l1_invalidate()
+// Starts the invalidation of this CU's vector L1, so that later loads read
+// L2; it has completed when the wave's vector memory counter reaches 0.
+__device__ __forceinline__ void l1_invalidate_issue()
 {
-  __asm__ volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
+  __asm__ volatile("buffer_inv sc0" ::: "memory");
 }
 
 // Memory of the resident solver: the solver block at `base`, addressed by
@@ -581,6 +582,11 @@ struct ResMem {
     if (threadIdx.x == 0) {
       unsigned *bar = (unsigned *)(base + obar);
       __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      // The L1 invalidation runs under the wait: one L1 per CU, this
+      // workgroup is alone on it, and until the other waves pass the
+      // barrier below nothing here loads through it (the polls are L2
+      // atomics, the error flag an agent-scope load).
+      l1_invalidate_issue();
       bool ok = true;
       for (unsigned n = 0;; n++) {
         if (__hip_atomic_fetch_add(bar, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= target)
@@ -593,7 +599,7 @@ struct ResMem {
         }
         __builtin_amdgcn_s_sleep(1);
       }
-      l1_invalidate();  // one L1 per CU, and this workgroup is alone on it
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the invalidation has completed
       *flag = ok ? 1.0 : 0.0;
     }
     __syncthreads();
