@@ -479,8 +479,6 @@ struct ResCtl {
                      // (res_layout.h): (r, s) pairs, then w (16 B per slot each)
   int nparts;        // partial slots per parity
   unsigned blk_bytes;  // bytes of the solver block from CgBatch::x (buffer range)
-  unsigned zero;     // 0; an addend the compiler cannot see, so that a poll
-                     // stays a read-modify-write atomic (executed in L2)
   unsigned spin;     // polls of one wait before it gives up (RES_SPIN_LIMIT)
 };
 
@@ -526,10 +524,10 @@ __device__ __forceinline__ void l1_invalidate_issue()
 // a uniform byte offset plus a per-lane byte offset (buffer accesses).  All
 // regions of a chain run on one XCD and exchange through its L2: plain
 // stores (the vector L1 writes through to L2), arrival counters updated by
-// L2 atomics and polled with a returning read-modify-write (a plain or
-// scope-0 load of the counter may be served by a stale L1 line, and the
-// compiler turns an atomic add of a known 0 into such a load), then an L1
-// invalidation before the reads (loads with cache policy sc0).
+// L2 atomics and polled with agent-scope loads (cache policy sc1: they
+// miss in the vector L1 and read L2; a plain or workgroup-scope load of the
+// counter may be served by a stale L1 line), then an L1 invalidation before
+// the reads (loads with cache policy sc0).
 typedef unsigned b64v __attribute__((ext_vector_type(2)));
 typedef unsigned b128v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -574,7 +572,7 @@ struct ResMem {
   // Arrive on the counter at byte offset `obar` and wait until it reaches
   // `target`; false when the wait timed out here or elsewhere (then the
   // error flag is set).  Every wave first waits for its own stores.
-  __device__ bool sync(unsigned obar, unsigned target, int *err, double *flag, unsigned zero,
+  __device__ bool sync(unsigned obar, unsigned target, int *err, double *flag,
                        unsigned spin) const
   {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores have completed
@@ -584,20 +582,20 @@ struct ResMem {
       __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       // The L1 invalidation runs under the wait: one L1 per CU, this
       // workgroup is alone on it, and until the other waves pass the
-      // barrier below nothing here loads through it (the polls are L2
-      // atomics, the error flag an agent-scope load).
+      // barrier below nothing here loads through it (the polls and the
+      // error flag are agent-scope loads, which miss in it).
       l1_invalidate_issue();
       bool ok = true;
+      // back-to-back polls, each one L2 round trip (a sleep between them,
+      // or a second poll in flight, measured slower)
       for (unsigned n = 0;; n++) {
-        if (__hip_atomic_fetch_add(bar, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= target)
-          break;
+        if (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
         if (n >= spin ||
             ((n & 255) == 255 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
           __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           ok = false;
           break;
         }
-        __builtin_amdgcn_s_sleep(1);
       }
       __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the invalidation has completed
       *flag = ok ? 1.0 : 0.0;
@@ -1095,7 +1093,7 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     }
     slot = ns;
     // 1. every region of the chain has finished iteration j
-    if (!M.sync(o_bar, (unsigned)(j + 2) * (unsigned)ntc, ctl.err, red + 36, ctl.zero, ctl.spin)) {
+    if (!M.sync(o_bar, (unsigned)(j + 2) * (unsigned)ntc, ctl.err, red + 36, ctl.spin)) {
       failed = true;
       return false;
     }
@@ -2208,7 +2206,6 @@ int resident_solve_s(th2d_ctx *ctx, th2d_buf &buf, int nch, const Geo *G,
   const size_t blk = (size_t)((char *)extra + extra_bytes - (char *)a.x);
   if (blk >= (size_t)1 << 31) return TH2D_ELAUNCH;  // buffer offsets are 32-bit
   ctl.blk_bytes = (unsigned)blk;
-  ctl.zero = 0;
   ctl.spin = ctx->res_spin_limit < 0 ? RES_SPIN_LIMIT : (unsigned)ctx->res_spin_limit;
   if (ctx->ktiming) TH2D_HIP(hipEventRecord(ctx->k0, ctx->stream));
   hipLaunchKernelGGL(k_zero_u32, dim3(std::min<int>(64, (int)(nzero + 255) / 256)), dim3(256), 0,
