@@ -899,6 +899,27 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     failed = red[37] == 0.0;  // the same decision in every region of the chain
   }
 
+  // The links of the wave's rows (K_t, K_x and the x neighbour's K_x) stay
+  // in registers for both stencils, loaded once (the barriers of the set-up
+  // wait above have ordered the LDS staging).  The mu != 0 instance with
+  // clipped regions of 64 rows keeps reading them from LDS: in registers it
+  // would spill.
+  constexpr bool LREG = MU0 || !CLIP || NO <= 4;
+  constexpr int NL = LREG ? NO : 1;
+  V2 lkt[NL], lkx[NL], lkxd[NL], lktd0;
+  const int cl0 = (wrow + 2) * W2 + (lane + 2);  // link index of (wrow, lane)
+  lktd0 = LKt[cl0 - W2];
+  if constexpr (LREG) {
+#pragma unroll
+    for (int q = 0; q < NO; q++) {
+      lkt[q] = LKt[cl0 + q * W2];
+      lkx[q] = LKx[cl0 + q * W2];
+      lkxd[q] = LKx[cl0 + q * W2 - 1];
+    }
+  }
+  auto link_t = [&](int q) { return LREG ? lkt[LREG ? q : 0] : LKt[cl0 + q * W2]; };
+  auto link_x = [&](int q) { return LREG ? lkx[LREG ? q : 0] : LKx[cl0 + q * W2]; };
+  auto link_xd = [&](int q) { return LREG ? lkxd[LREG ? q : 0] : LKx[cl0 + q * W2 - 1]; };
   const bool cont = ch.cscale > 0.0;
   double gam_old = cont ? ch.gam0 * ch.cscale * ch.cscale : 1.0;
   double alpha_old = cont ? ch.alpha0 : 1.0;
@@ -974,22 +995,20 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
       // r on the wave's rows is r[] (3a wrote the same values to R2); the
       // rows above and below come from LDS
       const V2 rtop = R2[c - W2], rbot = R2[c + NO * W2];
-      V2 ktd = LKt[c - W2];
+      V2 ktd = lktd0;
       V2 *const yo = Y1 + (wrow + 1) * W1 + (lane + 1);
       // unrolled, one element ahead: the loads of element q + 1 are issued
       // before the arithmetic of element q, which waits for its own loads only
-      V2 kt = LKt[c], kx = LKx[c], kxd = LKx[c - 1], rxp = R2[c + 1], rxm = R2[c - 1];
+      V2 rxp = R2[c + 1], rxm = R2[c - 1];
 #pragma unroll
       for (int q = 0; q < NO; q++) {
         const int cn = c + (q + 1) * W2;
-        V2 nkt, nkx, nkxd, nrxp, nrxm;
+        V2 nrxp, nrxm;
         if (q + 1 < NO) {
-          nkt = LKt[cn];
-          nkx = LKx[cn];
-          nkxd = LKx[cn - 1];
           nrxp = R2[cn + 1];
           nrxm = R2[cn - 1];
         }
+        const V2 kt = link_t(q), kx = link_x(q), kxd = link_xd(q);
         __builtin_amdgcn_sched_barrier(0);
         const V2 r0 = r[q], rtp = q + 1 < NO ? r[q + 1] : rbot, rtm = q > 0 ? r[q - 1] : rtop;
         // y = m r + e^mu K_t r(t+1) - e^-mu K_t(t-1)* r(t-1) + K_x r(x+1) - K_x(x-1)* r(x-1)
@@ -1002,9 +1021,6 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
         yv[q] = y;
         ktd = kt;
         if (q + 1 < NO) {
-          kt = nkt;
-          kx = nkx;
-          kxd = nkxd;
           rxp = nrxp;
           rxm = nrxm;
         }
@@ -1049,10 +1065,10 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
       int c2i = (wrow + 2) * W2 + (lane + 2);  // R2 / link index of (wrow, lane)
       // y on the wave's rows is yv[] (3b); the rows above and below from LDS
       const V2 ytop = Y1[c1 - W1], ybot = Y1[c1 + NO * W1];
-      V2 ktd = LKt[c2i - W2];
+      V2 ktd = lktd0;
 #pragma unroll
       for (int q = 0; q < NO; q++, c1 += W1, c2i += W2) {
-        const V2 kt = LKt[c2i], kx = LKx[c2i], kxd = LKx[c2i - 1];
+        const V2 kt = link_t(q), kx = link_x(q), kxd = link_xd(q);
         const V2 yxp = Y1[c1 + 1], yxm = Y1[c1 - 1];
         __builtin_amdgcn_sched_barrier(0);  // one LDS latency per element (as in 3b)
         const V2 y0 = yv[q], ytp = q + 1 < NO ? yv[q + 1] : ybot, ytm = q > 0 ? yv[q - 1] : ytop;
