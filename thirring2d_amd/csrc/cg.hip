@@ -654,6 +654,17 @@ __device__ __forceinline__ void res_ring_pos(int q, int &et, int &ex)
   }
 }
 
+// a b + c by the three-address FMA.  Written as a b + c, the compiler
+// picks the two-address form, whose destination must hold c first: where a
+// and c are loop-carried values and the result replaces a, that costs a
+// register copy of both every iteration.
+__device__ __forceinline__ float fma3(float a, float b, float c)
+{
+  float d;
+  __asm__("v_fma_f32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+
 // y += s a b and y += s conj(a) b for FP32 complex values, as chains of
 // fused multiply-adds into y: written as y + (a b - c d) the compiler may
 // not reassociate, and spends a separate add per term.  s = +-1 folds into
@@ -959,13 +970,15 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
       for (int q = 0; q < NO; q++) {
         const int lt = wrow + q, lx = lane;
         const bool own = !CLIP || ((inl >> q) & 1u);
-        const V2 sn = cadd(w[q], cscale(s[q], tb));
-        const V2 rn = csub(r[q], cscale(sn, ta));
-        R2[(lt + 2) * W2 + (lx + 2)] = rn;
+        // s, p and r by three-address FMAs (fma3), each result in the
+        // register of the value it replaces
+        const V2 sn = cmk<V2>(fma3(s[q].x, tb, w[q].x), fma3(s[q].y, tb, w[q].y));
         if (own) {
-          p[q] = cadd(r[q], cscale(p[q], tb));
+          p[q] = cmk<V2>(fma3(p[q].x, tb, r[q].x), fma3(p[q].y, tb, r[q].y));
           x[q] = cadd(x[q], cscale(p[q], ta));
         }
+        const V2 rn = cmk<V2>(fma3(sn.x, -ta, r[q].x), fma3(sn.y, -ta, r[q].y));
+        R2[(lt + 2) * W2 + (lx + 2)] = rn;
         r[q] = rn;
         s[q] = sn;
         M.st4(o_rs[ns], pso[q], f4v{rn.x, rn.y, sn.x, sn.y});  // nol: dropped
