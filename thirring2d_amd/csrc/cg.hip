@@ -692,14 +692,15 @@ __device__ __forceinline__ void cjfma(V &y, V a, V b, float s)
 // trips): pairs, quads (quad permutations), half-rows and rows (mirrors),
 // then rows 0+1 into row 1 and 2+3 into row 3 (broadcast of lane 15), and
 // rows 0..3 into row 3 (broadcast of lane 31); lane 63 then holds the total
-// in a fixed order, and readlane gives it to every lane.  Rows a
-// broadcast does not write add 0.
+// in a fixed order, and readlane gives it to every lane.  Lanes a
+// broadcast does not write keep an undefined value (no register has to be
+// cleared first): no such lane feeds lane 63.
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ double dpp_d(double v)
 {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 __device__ __forceinline__ double wave_total(double v)
