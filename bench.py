@@ -709,10 +709,10 @@ def main():
                 "flop_per_site_iteration": 88,
                 "hbm_GBs_measured": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
                 "note": ("FP32 vector-ALU arithmetic (the FP32 peak, 157.3 TFLOP/s, is the same "
-                         "for the vector ALUs and the matrix cores); the vectors stay in "
-                         "registers and the links in LDS, so HBM sees `traffic` bytes per "
-                         "launch; the rest of each iteration is the synchronisation of a "
-                         "chain's regions" if resident else
+                         "for the vector ALUs and the matrix cores); the vectors and the "
+                         "links of each wave's rows stay in registers (the halo in LDS), so "
+                         "HBM sees `traffic` bytes per launch; the rest of each iteration "
+                         "is the synchronisation of a chain's regions" if resident else
                          "streaming CG: every iteration reads and writes the vectors")}
     if args.only_headline:
         print(json.dumps({"metric": "batched mixed HMC (profiling run)", "value": tps,
