@@ -999,9 +999,9 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     __syncthreads();
     // 3b. y = M r_{j+1} on the 1-halo.  A wave takes the NO rows wrow + 1 ..
     // wrow + NO of the 1-halo frame at column lane + 1, its own sites: r(t),
-    // r(t +- 1) come from r[] and K_t(t-1) from the previous row, so an
-    // element loads its two x neighbours and three links from LDS (the
-    // rows above and below the wave's once).  Then the frame's first and
+    // r(t +- 1) come from r[] and the links from registers, so an element
+    // loads its two x neighbours from LDS (the rows above and below the
+    // wave's once).  Then the frame's first and
     // last rows and columns without the corners, at most one element per
     // thread.
     {
