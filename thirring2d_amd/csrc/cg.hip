@@ -1069,10 +1069,10 @@ __global__ void __launch_bounds__(S::NT) k_cg_resident(CgBatch<T> a, ResCtl ctl)
     __syncthreads();
     // 3c. w_{j+1} = M^dag y on the region; partials of iteration j+1 (a
     // thread's sites in FP32, then FP64).  y(t), y(t +- 1) come from yv[]
-    // (the rows above and below the wave's from LDS), K_t(t-1) from the
-    // previous row.  Elements
-    // outside the lattice (a region larger than the lattice) are computed
-    // too, to keep that chain, and their results dropped.
+    // (the rows above and below the wave's from LDS) and the links from
+    // registers, as in 3b.  Elements outside the lattice (a region larger
+    // than the lattice) are computed too, to keep that chain, and their
+    // results dropped.
     T gacc32 = 0, dacc32 = 0, racc32 = 0, sacc32 = 0;
     {
       int c1 = (wrow + 1) * W1 + (lane + 1);   // Y1 index of (wrow, lane)
