@@ -2336,7 +2336,15 @@ int resident_collect(th2d_ctx *ctx, int nch, const Geo *G, InnerResult *res, dou
   TH2D_TRY(th2d_host_reserve(ctx->resh, bytes));
   TH2D_HIP(hipMemcpyAsync(ctx->resh.ptr, ctx->res_pend_dev, bytes, hipMemcpyDeviceToHost,
                           ctx->stream));
-  TH2D_HIP(hipStreamSynchronize(ctx->stream));
+  // The host spins on an event instead of a blocking stream wait: the next
+  // pass is launched sooner (a failed device reports its error here too).
+  TH2D_HIP(hipEventRecord(ctx->sync_ev, ctx->stream));
+  {
+    hipError_t q;
+    while ((q = hipEventQuery(ctx->sync_ev)) == hipErrorNotReady) {
+    }
+    TH2D_HIP(q);
+  }
   ctx->hstage_off = 0;  // the stream is idle: every staged copy has been read
   if (rr) memcpy(rr, (char *)ctx->resh.ptr + rb + 64, sizeof(double) * nch);
   std::vector<CgResult> h(nch);

@@ -59,6 +59,7 @@ extern "C" int th2d_ctx_create(th2d_ctx **out, void *stream)
   TH2D_HIP(hipEventCreate(&c->k1));
   TH2D_HIP(hipEventCreate(&c->a0));
   TH2D_HIP(hipEventCreate(&c->a1));
+  TH2D_HIP(hipEventCreateWithFlags(&c->sync_ev, hipEventDisableTiming));
   TH2D_TRY(th2d::noalloc_scratch_init(c->device));
   *out = c;
   return TH2D_OK;
@@ -96,6 +97,7 @@ extern "C" int th2d_ctx_destroy(th2d_ctx *c)
   (void)hipEventDestroy(c->k1);
   (void)hipEventDestroy(c->a0);
   (void)hipEventDestroy(c->a1);
+  if (c->sync_ev) (void)hipEventDestroy(c->sync_ev);
   delete c;
   return TH2D_OK;
 }

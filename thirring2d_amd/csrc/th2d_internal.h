@@ -46,6 +46,7 @@ struct th2d_ctx {
   int ktiming = 0;
   hipEvent_t k0 = nullptr, k1 = nullptr;
   hipEvent_t a0 = nullptr, a1 = nullptr;  // autotuner
+  hipEvent_t sync_ev = nullptr;           // polled by a host wait that spins (no timing)
   double ktime_ms = 0.0;
   long klaunches = 0;
   double kbytes = 0.0;
