@@ -224,7 +224,7 @@ int th2d_set_cg_small(th2d_ctx *ctx, int on);
 int th2d_set_cg_resident_rows(th2d_ctx *ctx, int rows);
 /* Bounded waits of the resident solver.  Every wait between the regions
  * of a chain gives up after `polls` polls (polls < 0: the default, 2^20,
- * about 1 s) and ends every workgroup of the launch normally; the solve is
+ * at most about 1 s) and ends every workgroup of the launch normally; the solve is
  * then redone by the step kernel from the same right-hand sides, and
  * th2d_cg_sync_fallbacks counts how often that happened on this context.
  * A small limit forces the fallback from the host (used by the tests).

@@ -463,7 +463,7 @@ using RShape = Shape<64, 64, 512>;
 using RShapeH = Shape<32, 64, 512>;
 static_assert(RShape::TT == RES_T && RShape::TX == RES_T && RShapeH::TX == RES_T,
               "regions of the publication layout");
-constexpr unsigned RES_SPIN_LIMIT = 1u << 20;            // polls of one wait (about 1 s)
+constexpr unsigned RES_SPIN_LIMIT = 1u << 20;            // polls of one wait (at most about 1 s)
 constexpr int RES_PART_LANE = 4;  // partial slots a lane reads: at most 256 a chain (host check)
 constexpr unsigned kNoLoad = 0xFFFFFFFFu;                 // halo element no result needs
 
