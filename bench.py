@@ -41,7 +41,7 @@ FP32_PEAK_TFLOPS = 157.3
 # (profiles/k_cg_resident_hmc256_16chains_mixed.md), k_cg_step at 32 chains
 # (profiles/k_cg_step_hmc256_32chains_mixed.md)
 # key: (lattice size, chains, resident solver)
-TRAFFIC_BYTES = {(256, 32, False): 199.6e6, (256, 16, True): 165.4e6}
+TRAFFIC_BYTES = {(256, 32, False): 199.6e6, (256, 16, True): 165.1e6}
 
 
 def parse():
